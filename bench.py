@@ -1,0 +1,121 @@
+"""K-build benchmark on one MI355X.
+
+python bench.py [--config C] [--steps K] [--warmup W] [--gpus 1]
+
+A step is one truncated-Coulomb exchange (K) build through the libgint_*
+C-ABI: set_P, the recorded screened call stream of the workload, get_K
+(H2D of P and D2H of K included).  The metric counts surviving primitive
+quartet x lattice-cell slots (Fm[0] > 1e-12) per second; K-build seconds per
+step is ms_per_step.  The CPU baseline is the FP64 oracle on a strided
+sample of the same stream, on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector and FP64 matrix, dense
+PEAK_HBM_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--cpu-sample-sets", type=int, default=0, help="sets in the CPU sample (0: auto)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--max-mem", type=int, default=65536, help="device budget, MiB")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    from libgint_amd import driver as D, load
+
+    lib = load()
+    t0 = time.time()
+    s = D.System(a.config)
+    st = s.record()
+    t_rec = time.time() - t0
+    print(f"# config {a.config}: atoms {s.natom} nbf {s.nbf} cells {s.Ng} sets {st.n_sets} rows {st.n_rows} "
+          f"primitive quartets {st.n_prm} contracted quartets {st.n_qrtt} (recorded in {t_rec:.1f}s)",
+          file=sys.stderr, flush=True)
+    P = s.P()
+    h, table = D.setup_handle(lib, "libgint_", s, P=P, max_mem_mib=a.max_mem, stats=True, timing=True)
+    K = np.zeros(s.P_size)
+    dp = D._ptr
+
+    def step():
+        lib.libgint_set_P(h, dp(P), s.P_size)
+        st.replay(table, h)
+        lib.libgint_get_K(h, dp(K))
+
+    for w in range(a.warmup):
+        step()
+        print(f"# warmup {w} done", file=sys.stderr, flush=True)
+    lib.libgint_amd_synchronize(h)
+    lib.libgint_amd_reset_counters(h)
+    times = []
+    for k in range(a.steps):
+        lib.libgint_amd_synchronize(h)
+        t = time.perf_counter()
+        step()
+        lib.libgint_amd_synchronize(h)
+        times.append(time.perf_counter() - t)
+        print(f"# step {k} {times[-1]:.3f}s", file=sys.stderr, flush=True)
+    stats = D.stats(lib, "libgint_", h)
+    tm = (C_double9 := (__import__("ctypes").c_double * 9))()
+    lib.libgint_amd_timing(h, tm)
+    tot = sum(times)
+    slots = stats[1] / a.steps          # surviving slots per K-build
+    value = slots / (tot / a.steps)
+    kms = [tm[0] / a.steps, tm[1] / a.steps, tm[2] / a.steps]
+    kfl = [stats[6] / a.steps, tm[7] / a.steps, tm[8] / a.steps]
+    names = ["fm_vrr", "eco", "hrr_sph_ks"]
+    dom = int(np.argmax(kms))
+    achieved = kfl[dom] / (kms[dom] * 1e-3) / 1e12 if kms[dom] > 0 else 0.0
+    roof = {"kernel": names[dom], "bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": None,
+            "kernel_ms": {n: round(m, 3) for n, m in zip(names, kms)},
+            "kernel_tflops": {n: round(f / (m * 1e-3) / 1e12, 4) if m > 0 else 0 for n, f, m in zip(names, kfl, kms)}}
+
+    cpu = None
+    if not a.no_cpu:
+        import oracle
+
+        orc = oracle.load()
+        n_sample = a.cpu_sample_sets or max(1, min(st.n_sets, 20000))
+        stride = max(1, st.n_sets // n_sample)
+        ho, to = D.setup_handle(orc, "oracle_", s, P=P)
+        t = time.perf_counter()
+        st.replay(to, ho, 0, st.n_sets, stride)
+        tc = time.perf_counter() - t
+        so = D.stats(orc, "oracle_", ho)
+        orc.oracle_destroy_handle(ho)
+        cpu = {"value": round(so[1] / tc, 1), "unit": "quartets/s", "cores": 1, "kind": "port",
+               "sample": f"every {stride}th set quartet of the stream ({so[1]} surviving slots, {tc:.1f}s)"}
+
+    out = {
+        "metric": "K-build s/SCF step + quartets/s at max|dK|<=1e-8 Ha; p50/p99 request latency",
+        "value": round(value, 1), "unit": "quartets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(1e3 * tot / a.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"config {a.config}", "atoms": s.natom, "nbf": s.nbf, "cells": s.Ng,
+                   "set_quartets": st.n_sets, "primitive_quartets": st.n_prm, "contracted_quartets": st.n_qrtt,
+                   "surviving_slots": int(slots), "path": "plain per-class FP64"},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,115 @@
+/*
+ * libGint C-ABI for the MI355X (gfx950) K-build.
+ *
+ * The first group of entry points is the reference library's exported
+ * interface, one for one: same names, argument order and meaning, bound by
+ * CP2K through the reference's Fortran module (reference
+ * src/interface_libgint.F90:19-241).  Each declaration cites the reference
+ * function it replaces (src/extern_functions.cpp).  The second group is
+ * this library's own and is marked as such.
+ *
+ * Protocol (reference src/interface_libgint.F90:404-462): per set quartet,
+ * { {libgint_add_prm}* libgint_add_shell }* over the (n1,n2) images, then
+ * per angular-momentum combination libgint_add_qrt followed by nla*nlb*nlc*nld
+ * libgint_add_qrtt calls in (inla,inlb,inlc,inld) row-major order, then
+ * libgint_add_set.  libgint_init, libgint_set_P*, libgint_set_K* and
+ * libgint_get_K* are collective over the caller's OpenMP team.
+ *
+ * Errors: the reference returns void and exits on device errors; this
+ * library keeps the void signatures, prints a message and aborts on device
+ * errors and on arguments beyond its limits (l <= 3, <= 16 primitives per
+ * set, <= 4 contractions per shell, <= 256 lattice vectors).
+ */
+#ifndef LIBGINT_AMD_LIBGINT_H
+#define LIBGINT_AMD_LIBGINT_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference interface ---------------------------------------------- */
+
+/* src/extern_functions.cpp:9-15: new engine, one per OpenMP thread */
+void *libgint_create_handle(void);
+/* src/extern_functions.cpp:17-20: per-thread stream set-up (collective) */
+void libgint_init(void *handle);
+/* src/extern_functions.cpp:22-25: truncated Coulomb, C0 of 207 patches x ld_C0 */
+void libgint_set_Potential_Truncated(void *handle, double R_cut, double *C0, int ld_C0, int C0_size);
+/* src/extern_functions.cpp:27-32: K += fac * exchange */
+void libgint_set_hf_fac(void *handle, double fac);
+/* src/extern_functions.cpp:34-37: device memory budget per MPI rank, MiB */
+void libgint_set_max_mem(void *handle, int max_mem);
+/* src/extern_functions.cpp:40-44: density upload, also zeroes K (collective) */
+void libgint_set_P(void *handle, double *P, int P_size);
+/* src/extern_functions.cpp:46-50 */
+void libgint_set_P_polarized(void *handle, double *Pa, double *Pb, int P_size);
+/* src/extern_functions.cpp:52-55: start K from the given values (collective) */
+void libgint_set_K(void *handle, double *K, int K_size);
+/* src/extern_functions.cpp:57-60 */
+void libgint_set_K_polarized(void *handle, double *Ka, double *Kb, int K_size);
+/* src/extern_functions.cpp:62-65: finish all work, copy K to the host (collective) */
+void libgint_get_K(void *handle, double *K);
+/* src/extern_functions.cpp:67-70 */
+void libgint_get_K_polarized(void *handle, double *Ka, double *Kb);
+/* src/extern_functions.cpp:72-75: h and h^-1, 3x3 column-major, lattice vectors as columns */
+void libgint_set_cell(void *handle, bool periodic, double *cell_h, double *cell_i);
+/* src/extern_functions.cpp:77-80: lattice vectors (3 x nneighs, column-major) */
+void libgint_set_neighs(void *handle, double *neighs, int nneighs);
+/* src/extern_functions.cpp:82-88: set i at R with np exponents Z */
+void libgint_set_Atom(void *handle, int i, double *R, double *Z, int np);
+/* src/extern_functions.cpp:95-110: norm used to normalise one contraction */
+double compute_norm_psi(double *K, double *z, int np, int l);
+/* src/extern_functions.cpp:114-136: shell l of set i, nl x np coefficients */
+void libgint_set_Atom_L(void *handle, int i, int l, int nl, double *K);
+/* src/extern_functions.cpp:139-147: unused by the reference's caller; kept */
+void libgint_set_AtomInfo(void *handle, int i, double *R, double *Z, int np, int lmin, int Lmax, int *nl, double *K);
+/* src/extern_functions.cpp:149-152: one primitive quartet of the next add_shell */
+void libgint_add_prm(void *handle, int ipa, int ipb, int ipc, int ipd);
+/* src/extern_functions.cpp:154-157: set quartet (i,j,k,l) at images n1 (b) and n2 (d) */
+void libgint_add_shell(void *handle, int i, int j, int k, int l, int n1, int n2);
+/* src/extern_functions.cpp:159-162: no-op */
+void libgint_add_cell(void *handle);
+/* src/extern_functions.cpp:164-167: declares nla*nlb*nlc*nld quartets of class (la,lb,lc,ld) */
+void libgint_add_qrt(void *handle, int la, int lb, int lc, int ld, int nla, int nlb, int nlc, int nld);
+/* src/extern_functions.cpp:169-180: where one contracted quartet goes in K */
+void libgint_add_qrtt(void *handle, double symm_fac, int la, int lb, int lc, int ld, int inla, int inlb, int inlc,
+                      int inld, int ld_ac, int ld_ad, int ld_bc, int ld_bd, unsigned int offset_ac_L_set,
+                      unsigned int offset_ad_L_set, unsigned int offset_bc_L_set, unsigned int offset_bd_L_set,
+                      bool Tac, bool Tad, bool Tbc, bool Tbd);
+/* src/extern_functions.cpp:182-185: closes a set quartet */
+void libgint_add_set(void *handle);
+/* src/extern_functions.cpp:187-190: run all pending work now */
+void libgint_dispatch(void *handle);
+
+/* ---- this library's own (no counterpart in the reference) -------------- */
+
+/* release a handle (the reference never frees one) */
+void libgint_amd_destroy_handle(void *handle);
+/* wait for the handle's device work */
+void libgint_amd_synchronize(void *handle);
+/* count surviving (primitive quartet, cell) slots and their checksum */
+void libgint_amd_enable_stats(void *handle, int on);
+/* out[0] slots evaluated, [1] slots with Fm[0] > 1e-12, [2] primitive
+   quartets x classes, [3] contracted quartets, [4] checksum of the
+   surviving slots, [5] rows, [6] algorithmic FLOPs of fm_vrr */
+void libgint_amd_stats(void *handle, uint64_t out[8]);
+/* per-kernel device time with HIP events on the handle's stream */
+void libgint_amd_enable_timing(void *handle, int on);
+/* out[0..2] ms in fm_vrr / eco / hrr_sph_ks, out[3..5] launches,
+   out[6..8] algorithmic FLOPs of eco / hrr_sph_ks in out[7..8] (out[6] = 0;
+   the fm_vrr FLOPs are stats()[6]) */
+void libgint_amd_timing(void *handle, double out[9]);
+void libgint_amd_reset_counters(void *handle);
+/* evaluates the screening arithmetic on the GPU for a bitwise comparison
+   with the host: out[4i..4i+3] = exp(x), sqrt|x|, 1/x, x*y/(x+y) */
+int libgint_amd_math_selftest(const double *x, const double *y, int n, double *out);
+/* host evaluation of the same functions (same code) */
+void libgint_amd_math_host(const double *x, const double *y, int n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

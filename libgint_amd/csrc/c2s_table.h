@@ -28,28 +28,30 @@
 #define GINT_INV2SQRT3 0x1.279a74590331dp-2
 #define GINT_INVSQRT3  0x1.279a74590331dp-1
 
-static const double gint_c2s[GINT_C2S_SIZE] = {
-   /* l = 0 */
-   1.0,
-   /* l = 1 : y z x */
-   0.0, 1.0, 0.0,
-   0.0, 0.0, 1.0,
-   1.0, 0.0, 0.0,
-   /* l = 2 : columns xx xy xz yy yz zz */
-   0.0, 1.0, 0.0, 0.0, 0.0, 0.0,
-   0.0, 0.0, 0.0, 0.0, 1.0, 0.0,
-   -GINT_INV2SQRT3, 0.0, 0.0, -GINT_INV2SQRT3, 0.0, GINT_INVSQRT3,
-   0.0, 0.0, 1.0, 0.0, 0.0, 0.0,
-   0.5, 0.0, 0.0, -0.5, 0.0, 0.0,
-   /* l = 3 : columns xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz */
-   0.0, 0.612372436, 0.0, 0.0, 0.0, 0.0, -0.204124145, 0.0, 0.0, 0.0,
-   0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0,
-   0.0, -0.158113883, 0.0, 0.0, 0.0, 0.0, -0.158113883, 0.0, 0.632455532, 0.0,
-   0.0, 0.0, -0.387298335, 0.0, 0.0, 0.0, 0.0, -0.387298335, 0.0, 0.25819889,
-   -0.158113883, 0.0, 0.0, -0.158113883, 0.0, 0.632455532, 0.0, 0.0, 0.0, 0.0,
-   0.0, 0.0, 0.5, 0.0, 0.0, 0.0, 0.0, -0.5, 0.0, 0.0,
-   0.204124145, 0.0, 0.0, -0.612372436, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0,
-};
+/* the coefficients as a list, for host arrays and device constants alike */
+#define GINT_C2S_VALUES \
+   /* l = 0 */ \
+   1.0, \
+   /* l = 1 : y z x */ \
+   0.0, 1.0, 0.0, \
+   0.0, 0.0, 1.0, \
+   1.0, 0.0, 0.0, \
+   /* l = 2 : columns xx xy xz yy yz zz */ \
+   0.0, 1.0, 0.0, 0.0, 0.0, 0.0, \
+   0.0, 0.0, 0.0, 0.0, 1.0, 0.0, \
+   -GINT_INV2SQRT3, 0.0, 0.0, -GINT_INV2SQRT3, 0.0, GINT_INVSQRT3, \
+   0.0, 0.0, 1.0, 0.0, 0.0, 0.0, \
+   0.5, 0.0, 0.0, -0.5, 0.0, 0.0, \
+   /* l = 3 : columns xxx xxy xxz xyy xyz xzz yyy yyz yzz zzz */ \
+   0.0, 0.612372436, 0.0, 0.0, 0.0, 0.0, -0.204124145, 0.0, 0.0, 0.0, \
+   0.0, 0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0, \
+   0.0, -0.158113883, 0.0, 0.0, 0.0, 0.0, -0.158113883, 0.0, 0.632455532, 0.0, \
+   0.0, 0.0, -0.387298335, 0.0, 0.0, 0.0, 0.0, -0.387298335, 0.0, 0.25819889, \
+   -0.158113883, 0.0, 0.0, -0.158113883, 0.0, 0.632455532, 0.0, 0.0, 0.0, 0.0, \
+   0.0, 0.0, 0.5, 0.0, 0.0, 0.0, 0.0, -0.5, 0.0, 0.0, \
+   0.204124145, 0.0, 0.0, -0.612372436, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0
+
+static const double gint_c2s[GINT_C2S_SIZE] = {GINT_C2S_VALUES};
 
 static const int gint_c2s_off[4] = {GINT_C2S_OFF_0, GINT_C2S_OFF_1, GINT_C2S_OFF_2, GINT_C2S_OFF_3};
 

@@ -1,0 +1,257 @@
+"""Python side of the caller-side driver (libgint_amd/libgintdrv.so).
+
+Builds the synthetic systems of the workloads, records their screened
+libgint_* call streams, and replays a stream into any library that exports
+the libgint_* protocol: the product (libgint.so) or, in the tests, the CPU
+oracle (same call protocol, oracle_* names).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import DRV_PATH, TCG_PATH, load
+
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+_i64p = C.POINTER(C.c_int64)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("n_water", C.c_int), ("n_tm", C.c_int), ("box_A", C.c_double), ("image_shells", C.c_int),
+        ("basis", C.c_int), ("np_min", C.c_int), ("np_max", C.c_int), ("seed", C.c_uint64),
+        ("rcut_A", C.c_double), ("eps", C.c_double), ("min_oo_A", C.c_double), ("symmetric", C.c_int),
+        ("cell_A", C.c_double),
+    ]
+
+
+class AbiTable(C.Structure):
+    _fields_ = [(n, _vp) for n in ("add_prm", "add_shell", "add_qrt", "add_qrtt", "add_set", "set_Atom", "set_Atom_L")]
+
+
+# Workloads.  Config 0 has one cell; its cell edge (100 A) only has to make
+# the minimum image the identity for molecules placed in a 10 A cube.
+CONFIGS = {
+    0: dict(n_water=8, n_tm=0, box_A=10.0, image_shells=0, basis=0, np_min=1, np_max=6, seed=0),
+    1: dict(n_water=32, n_tm=0, box_A=12.4, image_shells=1, basis=0, np_min=1, np_max=8, seed=1),
+    2: dict(n_water=64, n_tm=8, box_A=15.6, image_shells=2, basis=0, np_min=1, np_max=8, seed=2),
+    4: dict(n_water=128, n_tm=0, box_A=15.6, image_shells=3, basis=4, np_min=1, np_max=16, seed=4),
+}
+
+_drv = None
+
+
+def drv():
+    global _drv
+    if _drv is None:
+        if not os.path.exists(DRV_PATH):
+            raise RuntimeError("libgint_amd/libgintdrv.so is not built: run __graft_entry__.build()")
+        d = C.CDLL(DRV_PATH)
+        sig = {
+            "gdrv_system_new": (_vp, [C.POINTER(Params)]),
+            "gdrv_system_free": (None, [_vp]),
+            "gdrv_system_info": (None, [_vp, _i64p]),
+            "gdrv_system_P": (None, [_vp, _dp]),
+            "gdrv_system_cell": (None, [_vp, _dp, _dp]),
+            "gdrv_system_neighs": (None, [_vp, _dp]),
+            "gdrv_system_rcut": (C.c_double, [_vp]),
+            "gdrv_system_atoms": (None, [_vp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "gdrv_kind_basis": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int), _dp]),
+            "gdrv_emit_basis": (None, [_vp, C.POINTER(AbiTable), _vp]),
+            "gdrv_stream_record": (_vp, [_vp, C.c_int]),
+            "gdrv_stream_info": (None, [_vp, _i64p]),
+            "gdrv_stream_free": (None, [_vp]),
+            "gdrv_stream_replay": (None, [_vp, _vp, C.POINTER(AbiTable), _vp, C.c_int64, C.c_int64, C.c_int64]),
+            "gdrv_symmetrize": (None, [_vp, _dp]),
+            "gdrv_dense": (None, [_vp, _dp, _dp]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(d, n)
+            f.restype = r
+            f.argtypes = a
+        _drv = d
+    return _drv
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def abi_table(lib, prefix="libgint_"):
+    t = AbiTable()
+    for name, _ in AbiTable._fields_:
+        setattr(t, name, C.cast(getattr(lib, prefix + name), _vp).value)
+    return t
+
+
+def tcg_table(nder=21):
+    """truncated-Coulomb coefficient table (207 x (nder+1)*105) and its ld"""
+    full = np.fromfile(TCG_PATH, dtype="<f8").reshape(207, 22 * 105)
+    c0 = np.ascontiguousarray(full[:, : (nder + 1) * 105])
+    return c0, (nder + 1) * 105
+
+
+class System:
+    def __init__(self, config=None, **kw):
+        p = dict(n_water=1, n_tm=0, box_A=10.0, image_shells=0, basis=0, np_min=1, np_max=4, seed=0,
+                 rcut_A=6.0, eps=1e-11, min_oo_A=2.4, symmetric=1, cell_A=0.0)
+        if config is not None:
+            p.update(CONFIGS[config])
+        p.update(kw)
+        self.params = dict(p)
+        self.periodic = p["image_shells"] > 0
+        if not self.periodic and p["cell_A"] == 0.0:
+            # molecules in a box_A cube, cell edge 100 A so the minimum image is the identity
+            p["cell_A"] = 100.0
+        self.params = dict(p)
+        self._p = Params(**{k: p[k] for k, _ in Params._fields_})
+        self.h = drv().gdrv_system_new(C.byref(self._p))
+        info = (C.c_int64 * 8)()
+        drv().gdrv_system_info(self.h, info)
+        self.natom, self.nbf, self.P_size, self.Ng = info[0], info[1], info[2], info[3]
+        self.np_kinds = (info[5], info[6], info[7])
+
+    def __del__(self):
+        try:
+            drv().gdrv_system_free(self.h)
+        except Exception:
+            pass
+
+    def P(self):
+        out = np.zeros(self.P_size)
+        drv().gdrv_system_P(self.h, _ptr(out))
+        return out
+
+    def cell(self):
+        h = np.zeros(9)
+        hi = np.zeros(9)
+        drv().gdrv_system_cell(self.h, _ptr(h), _ptr(hi))
+        return h, hi
+
+    def neighs(self):
+        g = np.zeros(3 * self.Ng)
+        drv().gdrv_system_neighs(self.h, _ptr(g))
+        return g
+
+    def rcut(self):
+        return drv().gdrv_system_rcut(self.h)
+
+    def record(self, nthreads=0):
+        return Stream(self, drv().gdrv_stream_record(self.h, nthreads))
+
+    def symmetrize(self, K):
+        K = np.ascontiguousarray(K, dtype=np.float64).copy()
+        drv().gdrv_symmetrize(self.h, _ptr(K))
+        return K
+
+    def dense(self, M):
+        D = np.zeros((self.nbf, self.nbf))
+        drv().gdrv_dense(self.h, _ptr(np.ascontiguousarray(M)), _ptr(D))
+        return D
+
+    def atoms(self):
+        R = np.zeros(3 * self.natom)
+        kind = np.zeros(self.natom, dtype=np.int32)
+        f0 = np.zeros(self.natom, dtype=np.int32)
+        drv().gdrv_system_atoms(self.h, _ptr(R), kind.ctypes.data_as(C.POINTER(C.c_int)), f0.ctypes.data_as(C.POINTER(C.c_int)))
+        return R.reshape(-1, 3), kind, f0
+
+    def kind_basis(self, kind):
+        Z = np.zeros(16)
+        lnl = np.zeros(16, dtype=np.int32)
+        K = np.zeros(16 * 4 * 8)
+        ns = drv().gdrv_kind_basis(self.h, kind, _ptr(Z), lnl.ctypes.data_as(C.POINTER(C.c_int)), _ptr(K))
+        np_ = self.np_kinds[kind]
+        shells = []
+        off = 0
+        for s in range(ns):
+            l, nl = int(lnl[2 * s]), int(lnl[2 * s + 1])
+            shells.append((l, nl, K[off: off + nl * np_].reshape(nl, np_).copy()))
+            off += nl * np_
+        return Z[:np_].copy(), shells
+
+
+class Stream:
+    def __init__(self, system, handle):
+        self.system = system
+        self.h = handle
+        info = (C.c_int64 * 5)()
+        drv().gdrv_stream_info(handle, info)
+        self.words, self.n_sets, self.n_rows, self.n_prm, self.n_qrtt = list(info)
+
+    def __del__(self):
+        try:
+            drv().gdrv_stream_free(self.h)
+        except Exception:
+            pass
+
+    def replay(self, table, handle, begin=0, end=None, stride=1):
+        if end is None:
+            end = self.n_sets
+        drv().gdrv_stream_replay(self.system.h, self.h, C.byref(table), handle, begin, end, stride)
+
+
+def setup_handle(lib, prefix, system, hf_fac=1.0, max_mem_mib=4096, truncated=True, P=None, Pb=None,
+                 stats=False, timing=False):
+    """create + configure a handle of `lib` (product or oracle) for `system`"""
+    f = lambda n: getattr(lib, prefix + n)
+    h = f("create_handle")()
+    f("init")(h)
+    if stats and prefix == "libgint_":
+        lib.libgint_amd_enable_stats(h, 1)
+    if timing and prefix == "libgint_":
+        lib.libgint_amd_enable_timing(h, 1)
+    if truncated:
+        c0, ld = tcg_table()
+        f("set_Potential_Truncated")(h, system.rcut(), _ptr(c0.ravel()), ld, c0.size)
+    f("set_hf_fac")(h, hf_fac)
+    f("set_max_mem")(h, max_mem_mib)
+    hm, hi = system.cell()
+    f("set_cell")(h, system.periodic, _ptr(hm), _ptr(hi))
+    g = system.neighs()
+    f("set_neighs")(h, _ptr(g), system.Ng)
+    table = abi_table(lib, prefix)
+    drv().gdrv_emit_basis(system.h, C.byref(table), h)
+    if P is None:
+        P = system.P()
+    if Pb is None:
+        f("set_P")(h, _ptr(P), len(P))
+    else:
+        f("set_P_polarized")(h, _ptr(P), _ptr(Pb), len(P))
+    return h, table
+
+
+def kbuild(lib, prefix, system, stream, begin=0, end=None, stride=1, K0=None, **kw):
+    """one K-build of (a sample of) the stream; returns K in block layout
+    (as accumulated by the library, before symmetrisation) and the handle.
+    With Pb in kw the build is spin-polarised and K is (Ka, Kb)."""
+    h, table = setup_handle(lib, prefix, system, **kw)
+    polar = kw.get("Pb") is not None
+    if K0 is not None:
+        if polar:
+            getattr(lib, prefix + "set_K_polarized")(h, _ptr(K0[0]), _ptr(K0[1]), system.P_size)
+        else:
+            getattr(lib, prefix + "set_K")(h, _ptr(K0), system.P_size)
+    stream.replay(table, h, begin, end, stride)
+    if polar:
+        Ka = np.zeros(system.P_size)
+        Kb = np.zeros(system.P_size)
+        getattr(lib, prefix + "get_K_polarized")(h, _ptr(Ka), _ptr(Kb))
+        return (Ka, Kb), h
+    K = np.zeros(system.P_size)
+    getattr(lib, prefix + "get_K")(h, _ptr(K))
+    return K, h
+
+
+def stats(lib, prefix, h):
+    out = (C.c_uint64 * 8)()
+    if prefix == "libgint_":
+        lib.libgint_amd_stats(h, out)
+    else:
+        lib.oracle_stats(h, out)
+    return list(out)
+
+
+def product():
+    return load()

@@ -18,54 +18,11 @@ def as_dp(a):
     return a.ctypes.data_as(dp)
 
 
-_oracle = None
-
-
 def oracle():
-    global _oracle
-    if _oracle is not None:
-        return _oracle
-    lib = C.CDLL(ORACLE_SO)
-    sig = {
-        "oracle_create_handle": (vp, []),
-        "oracle_destroy_handle": (None, [vp]),
-        "oracle_init": (None, [vp]),
-        "oracle_set_Potential_Truncated": (None, [vp, C.c_double, dp, C.c_int, C.c_int]),
-        "oracle_set_hf_fac": (None, [vp, C.c_double]),
-        "oracle_set_max_mem": (None, [vp, C.c_int]),
-        "oracle_set_P": (None, [vp, dp, C.c_int]),
-        "oracle_set_P_polarized": (None, [vp, dp, dp, C.c_int]),
-        "oracle_set_K": (None, [vp, dp, C.c_int]),
-        "oracle_set_K_polarized": (None, [vp, dp, dp, C.c_int]),
-        "oracle_get_K": (None, [vp, dp]),
-        "oracle_get_K_polarized": (None, [vp, dp, dp]),
-        "oracle_set_cell": (None, [vp, C.c_bool, dp, dp]),
-        "oracle_set_neighs": (None, [vp, dp, C.c_int]),
-        "oracle_set_Atom": (None, [vp, C.c_int, dp, dp, C.c_int]),
-        "oracle_set_Atom_L": (None, [vp, C.c_int, C.c_int, C.c_int, dp]),
-        "oracle_add_prm": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
-        "oracle_add_shell": (None, [vp] + [C.c_int] * 6),
-        "oracle_add_cell": (None, [vp]),
-        "oracle_add_qrt": (None, [vp] + [C.c_int] * 8),
-        "oracle_add_qrtt": (None, [vp, C.c_double] + [C.c_int] * 12 + [C.c_uint] * 4 + [C.c_bool] * 4),
-        "oracle_add_set": (None, [vp]),
-        "oracle_dispatch": (None, [vp]),
-        "oracle_norm_psi": (C.c_double, [dp, dp, C.c_int, C.c_int]),
-        "oracle_stats": (None, [vp, C.POINTER(C.c_uint64)]),
-        "oracle_exp": (C.c_double, [C.c_double]),
-        "oracle_boys": (None, [C.c_int, C.c_double, dp, C.c_double]),
-        "oracle_tcg": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, dp]),
-        "oracle_patch_grid": (None, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
-        "oracle_boys_table": (None, [dp]),
-        "oracle_slot_key": (C.c_uint64, [C.c_int] * 15),
-        "oracle_row_cart": (None, [vp] + [C.c_int] * 11 + [C.POINTER(C.c_uint32), dp]),
-    }
-    for name, (res, args) in sig.items():
-        f = getattr(lib, name)
-        f.restype = res
-        f.argtypes = args
-    _oracle = lib
-    return lib
+    """the CPU oracle (checker)"""
+    import oracle as _o
+
+    return _o.load()
 
 
 def tcg_table(nder=21):
