@@ -92,17 +92,32 @@ void libgint_amd_destroy_handle(void *handle);
 void libgint_amd_synchronize(void *handle);
 /* count surviving (primitive quartet, cell) slots and their checksum */
 void libgint_amd_enable_stats(void *handle, int on);
-/* out[0] slots evaluated, [1] slots with Fm[0] > 1e-12, [2] primitive
-   quartets x classes, [3] contracted quartets, [4] checksum of the
-   surviving slots, [5] rows, [6] algorithmic FLOPs of fm_vrr */
+/* out[0] (primitive quartet, cell, class) slots evaluated, [1] slots with
+   Fm[0] > 1e-12 (per class, the reference's recurrence work unit),
+   [2] primitive quartets x classes, [3] add_qrtt calls, [4] checksum of the
+   surviving slots, [5] rows (add_shell calls with primitives), [6] algorithmic
+   FLOPs of the fm_r kernel, [7] 0 */
 void libgint_amd_stats(void *handle, uint64_t out[8]);
 /* per-kernel device time with HIP events on the handle's stream */
 void libgint_amd_enable_timing(void *handle, int on);
-/* out[0..2] ms in fm_vrr / eco / hrr_sph_ks, out[3..5] launches,
-   out[6..8] algorithmic FLOPs of eco / hrr_sph_ks in out[7..8] (out[6] = 0;
-   the fm_vrr FLOPs are stats()[6]) */
-void libgint_amd_timing(void *handle, double out[9]);
+/* out[0..3] ms in fm_r / eco / hrr_sph_ks / fix-up, out[4..7] launches,
+   out[9], out[10] algorithmic FLOPs of eco and hrr_sph_ks (out[8] = 0: the
+   fm_r FLOPs are libgint_amd_stats out[6]; out[11] = 0) */
+void libgint_amd_timing(void *handle, double out[12]);
 void libgint_amd_reset_counters(void *handle);
+/* bit 0: route every surviving slot through the class-dependent screening
+   path (a test of that path; results are unchanged) */
+void libgint_amd_set_debug(void *handle, int flags);
+/* resident plans: the calls between plan_begin and plan_end are kept on the
+   device; plan_execute adds their exchange terms to K for the density of the
+   last set_P (between set_P and get_K, in place of the calls) */
+void libgint_amd_plan_begin(void *handle);
+void libgint_amd_plan_end(void *handle);
+void libgint_amd_plan_execute(void *handle);
+void libgint_amd_plan_clear(void *handle);
+/* out[0] resident bytes, [1] quartet types, [2] chunks, [3] set quartets,
+   [4] rows, [5] primitive quartets */
+void libgint_amd_plan_info(void *handle, uint64_t out[6]);
 /* evaluates the screening arithmetic on the GPU for a bitwise comparison
    with the host: out[4i..4i+3] = exp(x), sqrt|x|, 1/x, x*y/(x+y) */
 int libgint_amd_math_selftest(const double *x, const double *y, int n, double *out);

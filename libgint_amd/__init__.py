@@ -54,6 +54,12 @@ ABI = {
     "libgint_amd_enable_timing": (None, [_vp, _i]),
     "libgint_amd_timing": (None, [_vp, _dp]),
     "libgint_amd_reset_counters": (None, [_vp]),
+    "libgint_amd_set_debug": (None, [_vp, _i]),
+    "libgint_amd_plan_begin": (None, [_vp]),
+    "libgint_amd_plan_end": (None, [_vp]),
+    "libgint_amd_plan_execute": (None, [_vp]),
+    "libgint_amd_plan_clear": (None, [_vp]),
+    "libgint_amd_plan_info": (None, [_vp, C.POINTER(C.c_uint64)]),
     "libgint_amd_math_selftest": (_i, [_dp, _dp, _i, _dp]),
     "libgint_amd_math_host": (None, [_dp, _dp, _i, _dp]),
 }

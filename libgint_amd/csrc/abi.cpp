@@ -103,16 +103,22 @@ void libgint_amd_synchronize(void *h) { E(h)->synchronize(); }
 void libgint_amd_enable_stats(void *h, int on) { E(h)->enable_stats(on != 0); }
 void libgint_amd_stats(void *h, uint64_t out[8]) { E(h)->stats(out); }
 void libgint_amd_enable_timing(void *h, int on) { E(h)->enable_timing(on != 0); }
-void libgint_amd_timing(void *h, double out[9])
+void libgint_amd_timing(void *h, double out[12])
 {
    const gint::Timing t = E(h)->timing();
-   for (int i = 0; i < 3; i++) {
+   for (int i = 0; i < 4; i++) {
       out[i] = t.ms[i];
-      out[3 + i] = (double)t.launches[i];
-      out[6 + i] = t.flops[i];
+      out[4 + i] = (double)t.launches[i];
+      out[8 + i] = t.flops[i];
    }
 }
 void libgint_amd_reset_counters(void *h) { E(h)->reset_counters(); }
+void libgint_amd_set_debug(void *h, int flags) { E(h)->set_debug(flags); }
+void libgint_amd_plan_begin(void *h) { E(h)->plan_begin(); }
+void libgint_amd_plan_end(void *h) { E(h)->plan_end(); }
+void libgint_amd_plan_execute(void *h) { E(h)->plan_execute(); }
+void libgint_amd_plan_clear(void *h) { E(h)->plan_clear(); }
+void libgint_amd_plan_info(void *h, uint64_t out[6]) { E(h)->plan_info(out); }
 int libgint_amd_math_selftest(const double *x, const double *y, int n, double *out)
 {
    return gint::device_math_selftest(x, y, n, out);

@@ -1,7 +1,13 @@
 // Interface between the host engine (engine.cpp, built with g++ -fopenmp)
 // and the gfx950 kernels (kernels.hip, built with hipcc): record layouts in
-// HBM, the per-class plan passed by value to the kernels, and the launchers.
-// Plain types only.
+// HBM, the per-type and per-class plans, and the launchers.  Plain types
+// only.
+//
+// Work is organised by quartet *type*: the shell structure (l and number of
+// contractions of every shell) of the four sets of a set quartet.  All
+// shells of a set share its exponents, so the primitive work of a
+// (primitive quartet, lattice cell) slot is done once for every
+// angular-momentum class of the set quartet.
 #pragma once
 #include <stdint.h>
 #include <stddef.h>
@@ -10,99 +16,156 @@ namespace gint {
 
 constexpr int MAXL = 3;          // highest angular momentum of a shell (f)
 constexpr int MAXLL = 4 * MAXL;  // highest total angular momentum of a class
-constexpr int MAXNL = 4;         // contractions per shell and centre
+constexpr int MAXNL = 4;         // contractions per shell
 constexpr int MAXNP = 16;        // primitives per set
 constexpr int MAXNG = 256;       // lattice vectors
 constexpr int NCLASS = 256;      // (la,lb,lc,ld), each 0..3
+constexpr int MAXSH = 4;         // shells per set (distinct l)
+constexpr int MAXTC = MAXSH * MAXSH * MAXSH * MAXSH;  // classes per type
+constexpr int MAXHERM = 455;     // Hermite triples t+u+v <= 12
 
 inline constexpr int nco(int l) { return (l + 1) * (l + 2) / 2; }
 inline constexpr int nsph(int l) { return 2 * l + 1; }
-inline constexpr int class_index(int la, int lb, int lc, int ld) { return ((la * 4 + lb) * 4 + lc) * 4 + ld; }
+inline constexpr int nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+inline constexpr int class_code(int la, int lb, int lc, int ld) { return ((la * 4 + lb) * 4 + lc) * 4 + ld; }
 
-// One row: a set quartet (i,j,k,l) at bra/ket images (n1,n2) for one
-// angular-momentum class, with its primitive quartets.  All rows of one
-// set quartet and class share q_off (their integrals are summed).
+// One row: a set quartet at bra/ket images (n1,n2) with its primitive
+// quartets (one libgint_add_shell call).  Its contracted blocks sit at
+// (row - first row of the chunk) * TypePlan::ebytes in the contraction buffer.
 struct RowRec {
-   uint32_t prm_off;   // first primitive of the row in the batch
+   uint32_t prm_off;   // first primitive quartet of the row
    uint32_t n_prm;
-   uint32_t q_off;     // first contracted quartet of the row in the batch
-   uint32_t nlnp;      // nl[0..3] in 4-bit fields, then np[0..3]-1 in 4-bit fields
+   uint32_t group;     // its set quartet
    uint32_t n12;       // n1 | n2 << 16
-   uint32_t R[4];      // offsets of the four centres in the basis data
-   uint32_t Z[4];      // offsets of the exponents
-   uint32_t K[4];      // offsets of the normalised coefficients (nl x np)
-   uint32_t e_off;     // first contraction slot of the row in the contracted-block array
-   uint32_t set_ij;    // set indices i | j << 16 (survivor checksum only)
-   uint32_t set_kl;    // set indices k | l << 16
 };
-static_assert(sizeof(RowRec) == 80, "RowRec layout");
+static_assert(sizeof(RowRec) == 16, "RowRec layout");
 
-// One contracted quartet (one add_qrtt call): where its exchange terms go.
-struct KsRec {
-   uint32_t off[4];    // ac, ad, bc, bd block offsets into P / K
+// One set quartet (the work between two libgint_add_set calls).
+struct GroupRec {
+   uint32_t row0, row1;       // its rows
+   uint32_t R[4], Z[4];       // centre and exponent offsets in the basis data
+   uint32_t Kof[4][MAXSH];    // normalised coefficients of shell s of centre t (nl x np)
+   uint32_t np4;              // np-1 of the four sets, 4-bit fields
+   uint32_t set_ij, set_kl;   // set indices, 16-bit fields (survivor checksum only)
+   uint32_t pad[3];
+};
+static_assert(sizeof(GroupRec) == 128, "GroupRec layout");
+
+// Where the contracted quartets of one (set quartet, class) go in K: the
+// add_qrtt calls of the class compressed to offsets affine in the
+// contraction indices (inla, inlb, inlc, inld), or a reference to an
+// explicit list when the caller's offsets are not affine.
+struct KsGroup {
+   uint32_t base[4];   // ac ad bc bd block offsets of the first call
+   int32_t step[8];    // ac: by inla, inlc; ad: by inla, inld; bc: by inlb, inlc; bd: by inlb, inld
    uint32_t ld01;      // ld_ac | ld_ad << 16
    uint32_t ld23;      // ld_bc | ld_bd << 16
-   uint32_t flags;     // bit t: transpose flag of block t; bits 4..6: symmetry code
-   uint32_t row0;      // first row of its set quartet
-   uint32_t row1;      // one past the last row
-   uint32_t l;         // contraction combination index inside the quartet
-   uint32_t pad[2];
+   uint32_t flags;     // bits 0-3 transposes, 4-6 symmetry code, bit 8 explicit, bits 16-24 calls
+   uint32_t expl;      // first explicit record when bit 8 is set
 };
-static_assert(sizeof(KsRec) == 48, "KsRec layout");
+static_assert(sizeof(KsGroup) == 64, "KsGroup layout");
 
-// Everything a kernel needs to know about one class; passed by value.
+constexpr uint32_t KS_EXPLICIT = 1u << 8;
+inline uint32_t ks_count(uint32_t flags) { return flags >> 16; }
+
+// one add_qrtt call, kept when a (set quartet, class) is not affine
+struct KsRec {
+   uint32_t off[4];
+   uint32_t ld01, ld23, flags, pad;
+};
+static_assert(sizeof(KsRec) == 32, "KsRec layout");
+
+// Plan of one angular-momentum class: HRR layers, cart->sph, sizes.
 struct ClassPlan {
    int la, lb, lc, ld;
-   int L, LA, LC;            // L = la+lb+lc+ld, LA = la+lb, LC = lc+ld
-   int ncp;                  // size of the (e0|f0) block set, e in [la,LA], f in [lc,LC]
-   int cpoff[7][7];          // offset of block (e,f) inside the set
-   int vws;                  // VRR workspace doubles per slot
-   int voff[7][7];           // offset of VRR block (a,c) ([m][ia][ic], m = 0..L-a-c)
-   int amin[7];              // lowest a needed at ket level c
-   int Nc, Ns;               // cartesian / spherical sizes of the class
-   // HRR layers.  Ket layer dd (0..ld) holds blocks (e, c) for e in [la,LA],
-   // c in [lc, LC-dd], layout [ie][ic][id], e outer; layer 0 is the (e0|f0)
-   // set itself.  Bra layer bb (0..lb) holds blocks a in [la, LA-bb], layout
-   // [ia][ib][icd]; bra layer 0 is the last ket layer.
-   int koff[4][7][7];        // offset of ket block (e,c) in layer dd
+   int L, LA, LC;
+   int ncp;                  // its (e0|f0) block set, e in [la,LA], f in [lc,LC], e outer
+   int cpoff[7][7];
+   int Nc, Ns;
+   int koff[4][7][7];        // ket HRR layer dd: block (e,c), e in [la,LA], c in [lc,LC-dd]
    int ksize[4];
-   int boff[4][7];           // offset of bra block a in layer bb
+   int boff[4][7];           // bra HRR layer bb: block a in [la, LA-bb]
    int bsize[4];
    int ycap;                 // capacity of one layer buffer (>= ncp, Nc, every layer)
-   int hrr_ws;               // HRR workspace doubles per quartet: 3 ycap + Nc
-   // algorithmic FLOPs fixed by the reference recurrence chain (not by
-   // this implementation): per surviving slot, per contracted quartet
-   int fm_flops;             // 52 + 211 (L+1)
-   int vrr_flops;            // reference plan VRR ops, weights 3/8/3/8/11/6 per element
+   int hrr_ws;               // HRR workspace doubles per workgroup: 3 ycap + Nc
+   // algorithmic FLOPs of the reference chain for this class
+   int fm_flops;             // 52 + 211 (L+1) per surviving slot
+   int vrr_flops;            // reference plan VRR (weights 3/8/3/8/11/6 per element)
    int hrr_flops;            // 2 per HRR element
    int sph_flops;            // 2 m n k over the four stages
 };
 
-// Device-resident, per-process constant tables.
+// Plan of one quartet type.  Blocks (e0|f0) with e in [LA0,LA], f in
+// [LC0,LC] (e outer, layout [ie][jf] inside a block) are produced per
+// primitive quartet, summed over lattice cells; each class reads its own
+// sub-range of them.
+struct TypePlan {
+   int nsh[4];
+   int l[4][MAXSH];
+   int nl[4][MAXSH];
+   int nclass;               // nsh0*nsh1*nsh2*nsh3, sa outer
+   int LA0, LA, LC0, LC, L;
+   int ncp;
+   int cpoff[7][7];
+   int nkc;                  // ket cartesian components, f in [LC0,LC]
+   int kcoff[7];             // first of shell f among them
+   int nbh;                  // bra Hermite triples t+u+v <= LA
+   int nh;                   // Hermite triples t+u+v <= L
+   int has00;                // LA0 == 0 && LC0 == 0
+   int nL;                   // distinct class L values
+   int Ls[16];
+   int ebytes;               // contracted doubles per row: sum over classes nlabcd ncp
+   int cls_code[MAXTC];      // class_code(la,lb,lc,ld) of class k
+   int cls_sh[MAXTC];        // sa | sb<<4 | sc<<8 | sd<<12
+   int cls_gate[MAXTC];      // index into Ls
+   int cls_nl[MAXTC];        // nla*nlb*nlc*nld
+   int cls_ncp[MAXTC];
+   int cls_eoff[MAXTC];      // offset of class k inside a row's contracted doubles
+   unsigned long long cls_mult[MAXTC];   // checksum multiplier of class k (odd)
+   unsigned long long mult_all;          // sum of cls_mult
+   // algorithmic FLOPs of this library's kernels (Hermite path)
+   long long surv_flops;     // per slot that survives for some class: G_m and the R recursion
+   long long prim_flops;     // per primitive quartet: Hermite -> cartesian transform
+   long long ref_slot_flops; // per slot, summed over classes: the reference's Fm + VRR
+};
+
+// Device-resident constant tables.
 struct DevTables {
-   const double *C0;        // patch coefficients, ld_C0 per patch
+   const double *C0;
    int ld_C0;
    const int *grid_lo, *grid_hi;
-   const double *bw;        // 207 x 4
-   const double *ftable;    // 121 x 28 Boys table
-   const double *cell;      // 18
-   const double *neighs;    // 3 x Ng
+   const double *bw;
+   const double *ftable;
+   const double *cell;
+   const double *neighs;
    int Ng;
    int truncated;
    double R_cut;
 };
 
-// One class batch resident in HBM.
+// One chunk of one type's work, resident in HBM.  Primitive, row and group
+// indices are absolute within the type's arrays; ac0 and eco are indexed
+// from the chunk's first primitive and first row.
 struct BatchDev {
+   const TypePlan *tp;        // device copy
    const RowRec *rows;
-   const uint32_t *prm_row;   // row of each primitive
+   const GroupRec *groups;
+   const uint32_t *prm_row;   // row of each primitive quartet
    const uint32_t *prm_ip;    // ipa | ipb << 4 | ipc << 8 | ipd << 12
-   const KsRec *ks;
-   const double *data;        // basis data (positions, exponents, coefficients)
-   double *ac0;               // [n_prm][ncp]
-   double *eco;               // per row [nlabcd][ncp]
-   uint32_t n_rows, n_prm, n_q;
-   unsigned long long *stats; // [0] surviving slots, [1] slot checksum, [2] fm+vrr FLOPs (may be null)
+   const KsGroup *ksg;        // [group][class]
+   const KsRec *ksx;
+   const double *data;
+   double *ac0;               // [primitive - p0][ncp]
+   double *eco;               // [row - r0][ebytes]
+   uint32_t p0, p1;           // primitives of the chunk
+   uint32_t r0, r1;           // rows
+   uint32_t g0, g1;           // groups
+   unsigned long long *stats; // [0] surviving (slot, class) pairs, [1] checksum, [2] K1 FLOPs (nullable)
+   uint32_t *fix;             // mixed-gate slots: (primitive, n3 | mask << 8)
+   uint32_t *fix_count;       // [0] entries, [1] overflow
+   uint32_t fix_cap;
+   int force_fix;             // route every surviving slot through the fix-up path (test hook)
+   double *fix_ws;            // fix-up workspace, ncp doubles per entry
 };
 
 struct KTargets {
@@ -112,32 +175,28 @@ struct KTargets {
    double hf_fac;
 };
 
-// Scratch in HBM for classes whose per-wavefront workspace exceeds the LDS
-// budget; owned by the caller (one per stream).
 struct Scratch {
    double *ptr;
    size_t doubles;
 };
 
-// launchers (kernels.hip); all asynchronous on `stream`, return 0 or a
-// negative code (-1 launch failure, -2 scratch too small)
-int launch_fm_vrr(const ClassPlan &cp, const DevTables &t, const BatchDev &b, Scratch s, void *stream);
-// contraction of rows [row_begin,row_end) over the primitives [prm_begin,prm_end)
-// whose (e0|f0) sums sit in b.ac0 (indexed from prm_begin); accumulates
-int launch_eco_range(const ClassPlan &cp, const BatchDev &b, uint32_t row_begin, uint32_t row_end,
-                     uint32_t prm_begin, uint32_t prm_end, void *stream);
-int launch_hrr_sph_ks(const ClassPlan &cp, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s,
-                      void *stream);
-// workgroups that get an HBM workspace slice when a class does not fit in LDS
 constexpr unsigned SCRATCH_GRID = 2048;
-// HBM scratch doubles the class needs (0 if its workspaces fit in LDS)
+
+// launchers (kernels.hip), asynchronous on `stream`; 0 or a negative code
+int launch_fm_r(const TypePlan &tp, const DevTables &t, const BatchDev &b, void *stream);
+int launch_eco(const TypePlan &tp, const BatchDev &b, void *stream);
+int launch_fixup(const TypePlan &tp, const DevTables &t, const BatchDev &b, uint32_t n_fix, void *stream);
+int launch_hrr_sph_ks(const ClassPlan &cp, int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b,
+                      const KTargets &kt, Scratch s, void *stream);
 size_t scratch_doubles(const ClassPlan &cp);
-// fill a ClassPlan (plan.cpp)
+// LDS bytes of the fused kernel for a type (0 if it does not fit)
+size_t fm_r_lds_bytes(const TypePlan &tp);
+
 ClassPlan make_plan(int la, int lb, int lc, int ld);
-// 121 x 28 Boys table (boys_table.cpp)
+// fills everything from nsh, l and nl
+void make_type_plan(TypePlan &tp);
+uint64_t class_mult(int code);
 void boys_table(double *ftable);
-// device self-test of the scalar functions that decide screening:
-// out[4i..4i+3] = {gint_exp(x), sqrt(|x|), 1/x, x*y/(x+y)} evaluated on the GPU
 int device_math_selftest(const double *x, const double *y, int n, double *out);
 
 }  // namespace gint

@@ -100,7 +100,7 @@ struct GdrvParams {
    int n_tm;
    double box_A;
    int image_shells;   // 0: 1 cell, 1: 27, 2: 125, 3: 251 (closed shells |g|^2 <= 13)
-   int basis;          // 0: O 3s2p1d / H 2s1p; 4: O 4s3p2d1f / H 3s2p1d
+   int basis;          // 0: O 3s2p1d / H 2s1p; 1: O 2s1p / H 1s; 4: O 4s3p2d1f / H 3s2p1d
    int np_min, np_max;
    uint64_t seed;
    double rcut_A;
@@ -195,6 +195,9 @@ void build(GdrvSystem &S)
    if (p.basis == 4) {
       S.kinds.push_back(make_basis(rng, {{0, 4}, {1, 3}, {2, 2}, {3, 1}}, rng.range(p.np_min, p.np_max)));
       S.kinds.push_back(make_basis(rng, {{0, 3}, {1, 2}, {2, 1}}, rng.range(p.np_min, p.np_max)));
+   } else if (p.basis == 1) {
+      S.kinds.push_back(make_basis(rng, {{0, 2}, {1, 1}}, rng.range(p.np_min, p.np_max)));
+      S.kinds.push_back(make_basis(rng, {{0, 1}}, rng.range(p.np_min, p.np_max)));
    } else {
       S.kinds.push_back(make_basis(rng, {{0, 3}, {1, 2}, {2, 1}}, rng.range(p.np_min, p.np_max)));
       S.kinds.push_back(make_basis(rng, {{0, 2}, {1, 1}}, rng.range(p.np_min, p.np_max)));

@@ -18,9 +18,9 @@
 
 namespace gint {
 
-void fatal(const char *what, int code)
+void fatal(const char *what, long long code)
 {
-   std::fprintf(stderr, "libgint: %s (code %d)\n", what, code);
+   std::fprintf(stderr, "libgint: %s (%lld)\n", what, code);
    std::fflush(stderr);
    std::abort();
 }
@@ -61,6 +61,37 @@ uint32_t DataStore::add(const double *v, int n)
    return off;
 }
 
+template <class T>
+static void free_vec(std::vector<T> &v)
+{
+   std::vector<T>().swap(v);
+}
+
+void TypeQueue::release()
+{
+   free_vec(rows);
+   free_vec(prm_ip);
+   free_vec(prm_row);
+   free_vec(groups);
+   free_vec(ksg);
+   free_vec(ksx);
+}
+
+void TypeDev::release()
+{
+   void *p[6] = {rows, groups, prm_ip, prm_row, ksg, ksx};
+   for (void *x : p)
+      if (x) (void)hipFree(x);
+   rows = nullptr;
+   groups = nullptr;
+   prm_ip = prm_row = nullptr;
+   ksg = nullptr;
+   ksx = nullptr;
+   for (auto &c : cap) c = 0;
+   n_rows = n_groups = n_prm = n_ksx = 0;
+   chunks.clear();
+}
+
 // ---------------------------------------------------------------------------
 // team-shared objects: streams per OpenMP thread (reference
 // src/libGint.cpp:26,37-60) and the device P / K of the current step
@@ -99,19 +130,16 @@ Engine::Engine()
 
 Engine::~Engine()
 {
-   if (d_data_) (void)hipFree(d_data_);
-   if (d_C0_) (void)hipFree(d_C0_);
-   if (d_grid_) (void)hipFree(d_grid_);
-   if (d_bw_) (void)hipFree(d_bw_);
-   if (d_ftab_) (void)hipFree(d_ftab_);
-   if (d_cell_) (void)hipFree(d_cell_);
-   if (d_neighs_) (void)hipFree(d_neighs_);
-   if (d_arena_) (void)hipFree(d_arena_);
-   if (h_stage_) (void)hipHostFree(h_stage_);
-   if (d_scratch_) (void)hipFree(d_scratch_);
-   if (d_stats_) (void)hipFree(d_stats_);
-   for (auto &e : ev_)
-      if (e) (void)hipEventDestroy((hipEvent_t)e);
+   if (stream_) (void)hipStreamSynchronize((hipStream_t)stream_);
+   for (auto &d : stream_dev_) d.release();
+   for (auto &d : plan_dev_) d.release();
+   for (auto &ti : types_)
+      if (ti.d_plan) (void)hipFree(ti.d_plan);
+   void *bufs[] = {d_data_, d_C0_, d_grid_, d_bw_, d_ftab_, d_cell_, d_neighs_, d_work_,
+                   d_fix_, d_fix_ws_, d_scratch_, d_stats_, d_fixcnt_};
+   for (void *b : bufs)
+      if (b) (void)hipFree(b);
+   for (void *e : ev_pool_) (void)hipEventDestroy((hipEvent_t)e);
 }
 
 void Engine::init()
@@ -130,6 +158,7 @@ void Engine::init()
 void Engine::set_potential_truncated(double R_cut, const double *C0, int ld_C0, int C0_size)
 {
    if (ld_C0 % TCG_NCOEF != 0 || C0_size < ld_C0 * TCG_NPATCH) fatal("set_Potential_Truncated: bad C0 table shape", ld_C0);
+   if (ld_C0 / TCG_NCOEF < MAXLL + 1) fatal("set_Potential_Truncated: the table needs derivative orders 0..12", ld_C0);
    truncated_ = true;
    R_cut_ = R_cut;
    ld_C0_ = ld_C0;
@@ -161,7 +190,7 @@ void Engine::set_neighs(const double *g, int n)
 // reference src/libGint.cpp:117-148
 void Engine::set_atom(int i, const double *R, const double *Z, int np)
 {
-   if (i < 0) fatal("set_Atom: negative set index", i);
+   if (i < 0 || i > 0xffff) fatal("set_Atom: set index out of 0..65535", i);
    if (np < 1 || np > MAXNP) fatal("set_Atom: 1..16 primitives per set", np);
    if (i >= (int)sets_.size()) sets_.resize(i + 1);
    SetInfo &s = sets_[i];
@@ -171,6 +200,8 @@ void Engine::set_atom(int i, const double *R, const double *Z, int np)
    s.l.clear();
    s.nl.clear();
    s.K.clear();
+   s.sig = -1;
+   last_type_ = -1;
 }
 
 void Engine::set_atom_L(int i, int l, int nl, const double *K)
@@ -179,9 +210,59 @@ void Engine::set_atom_L(int i, int l, int nl, const double *K)
    if (l < 0 || l > MAXL) fatal("set_Atom_L: l must be 0..3", l);
    if (nl < 1 || nl > MAXNL) fatal("set_Atom_L: 1..4 contractions per shell", nl);
    SetInfo &s = sets_[i];
+   for (int x : s.l)
+      if (x == l) fatal("set_Atom_L: one shell per l and set", l);
    s.l.push_back(l);
    s.nl.push_back(nl);
    s.K.push_back(store_.add(K, nl * s.np));
+   s.sig = -1;
+   last_type_ = -1;
+}
+
+int Engine::set_sig(int i)
+{
+   SetInfo &s = sets_[i];
+   if (s.sig >= 0) return s.sig;
+   if (s.l.empty()) fatal("set without shells (set_Atom_L) used in add_shell", i);
+   auto key = std::make_pair(s.l, s.nl);
+   auto it = sig_ids_.find(key);
+   if (it == sig_ids_.end()) it = sig_ids_.emplace(key, (int)sig_ids_.size()).first;
+   s.sig = it->second;
+   return s.sig;
+}
+
+int Engine::type_of(int i, int j, int k, int l)
+{
+   const std::array<int, 4> sets = {i, j, k, l};
+   if (sets == last_sets_ && last_type_ >= 0) return last_type_;
+   const std::array<int, 4> sig = {set_sig(i), set_sig(j), set_sig(k), set_sig(l)};
+   auto it = type_ids_.find(sig);
+   if (it == type_ids_.end()) {
+      TypeInfo ti;
+      ti.sig = sig;
+      std::memset(&ti.plan, 0, sizeof ti.plan);
+      const SetInfo *S[4] = {&sets_[i], &sets_[j], &sets_[k], &sets_[l]};
+      for (int t = 0; t < 4; t++) {
+         ti.plan.nsh[t] = (int)S[t]->l.size();
+         if (ti.plan.nsh[t] > MAXSH) fatal("more than 4 shells in a set", ti.plan.nsh[t]);
+         for (int s = 0; s < ti.plan.nsh[t]; s++) {
+            ti.plan.l[t][s] = S[t]->l[s];
+            ti.plan.nl[t][s] = S[t]->nl[s];
+         }
+      }
+      make_type_plan(ti.plan);
+      if (!fm_r_lds_bytes(ti.plan)) fatal("quartet type exceeds the on-chip budget", ti.plan.L);
+      for (int c = 0; c < ti.plan.nclass; c++) {
+         const int code = ti.plan.cls_code[c];
+         ti.cplans.push_back(make_plan(code / 64, (code / 16) % 4, (code / 4) % 4, code % 4));
+      }
+      types_.push_back(ti);
+      q_.emplace_back();
+      it = type_ids_.emplace(sig, (int)types_.size() - 1).first;
+   }
+   last_sets_ = sets;
+   last_type_ = it->second;
+   return last_type_;
 }
 
 // ---------------------------------------------------------------------------
@@ -261,7 +342,14 @@ void Engine::get_K(double *Ka, double *Kb)
 
 void Engine::synchronize()
 {
-   if (stream_) HIPCHECK(hipStreamSynchronize((hipStream_t)stream_));
+   if (!stream_) return;
+   HIPCHECK(hipStreamSynchronize((hipStream_t)stream_));
+   harvest_events();
+   if (d_fixcnt_) {
+      uint32_t c[2];
+      HIPCHECK(hipMemcpy(c, d_fixcnt_, sizeof c, hipMemcpyDeviceToHost));
+      if (c[1]) fatal("more slots with class-dependent screening in one chunk than the fix-up list holds", fix_cap_);
+   }
 }
 
 // ---------------------------------------------------------------------------
@@ -276,58 +364,48 @@ void Engine::add_prm(int ipa, int ipb, int ipc, int ipd)
 
 void Engine::add_shell(int i, int j, int k, int l, int n1, int n2)
 {
-   if (prm_tmp_.empty()) return;
+   if (prm_tmp_.empty()) return;   // reference src/libGint.cpp:159
    const int ns = (int)sets_.size();
    if (i < 0 || j < 0 || k < 0 || l < 0 || i >= ns || j >= ns || k >= ns || l >= ns) fatal("add_shell: unknown set", i);
    const int Ng = (int)neighs_.size() / 3;
    if (n1 < 0 || n2 < 0 || n1 >= Ng || n2 >= Ng) fatal("add_shell: cell index beyond set_neighs", n1 > n2 ? n1 : n2);
    const SetInfo *S[4] = {&sets_[i], &sets_[j], &sets_[k], &sets_[l]};
-   for (int t = 0; t < 4; t++)
-      for (uint32_t ip : prm_tmp_) {
-         const int v = (ip >> (4 * t)) & 15;
-         if (v >= S[t]->np) fatal("add_prm: primitive index beyond the set's primitives", v);
+   for (uint32_t ip : prm_tmp_)
+      for (int t = 0; t < 4; t++)
+         if ((int)((ip >> (4 * t)) & 15) >= S[t]->np) fatal("add_prm: primitive index beyond the set's primitives", t);
+   const int type = type_of(i, j, k, l);
+   TypeQueue &Q = q_[type];
+   if (cur_type_ < 0) {
+      cur_type_ = type;
+      cur_sets_ = {i, j, k, l};
+      GroupRec g;
+      std::memset(&g, 0, sizeof g);
+      g.row0 = (uint32_t)Q.rows.size();
+      for (int t = 0; t < 4; t++) {
+         g.R[t] = S[t]->R;
+         g.Z[t] = S[t]->Z;
+         for (size_t s = 0; s < S[t]->K.size(); s++) g.Kof[t][s] = S[t]->K[s];
       }
-   const uint32_t np_bits = (uint32_t)(S[0]->np - 1) << 16 | (uint32_t)(S[1]->np - 1) << 20 |
-                            (uint32_t)(S[2]->np - 1) << 24 | (uint32_t)(S[3]->np - 1) << 28;
-   for (size_t sa = 0; sa < S[0]->l.size(); sa++)
-      for (size_t sb = 0; sb < S[1]->l.size(); sb++)
-         for (size_t sc = 0; sc < S[2]->l.size(); sc++)
-            for (size_t sd = 0; sd < S[3]->l.size(); sd++) {
-               const int c = class_index(S[0]->l[sa], S[1]->l[sb], S[2]->l[sc], S[3]->l[sd]);
-               ClassQueue &Q = q_[c];
-               if (!Q.touched) {
-                  Q.touched = true;
-                  Q.set_row0 = Q.rows.size();
-                  Q.set_prm0 = Q.prm_ip.size();
-                  Q.set_ks0 = Q.ks.size();
-                  Q.set_nq = 0;
-                  Q.set_eslots0 = Q.e_slots;
-                  set_classes_.push_back(c);
-               }
-               const int nla = S[0]->nl[sa], nlb = S[1]->nl[sb], nlc = S[2]->nl[sc], nld = S[3]->nl[sd];
-               RowRec r;
-               r.prm_off = (uint32_t)Q.prm_ip.size();
-               r.n_prm = (uint32_t)prm_tmp_.size();
-               r.q_off = Q.n_q + Q.set_nq;
-               r.nlnp = (uint32_t)nla | (uint32_t)nlb << 4 | (uint32_t)nlc << 8 | (uint32_t)nld << 12 | np_bits;
-               r.n12 = (uint32_t)n1 | (uint32_t)n2 << 16;
-               for (int t = 0; t < 4; t++) {
-                  r.R[t] = S[t]->R;
-                  r.Z[t] = S[t]->Z;
-               }
-               r.K[0] = S[0]->K[sa];
-               r.K[1] = S[1]->K[sb];
-               r.K[2] = S[2]->K[sc];
-               r.K[3] = S[3]->K[sd];
-               r.e_off = Q.e_slots;
-               r.set_ij = (uint32_t)(i & 0xffff) | (uint32_t)(j & 0xffff) << 16;
-               r.set_kl = (uint32_t)(k & 0xffff) | (uint32_t)(l & 0xffff) << 16;
-               Q.e_slots += (uint32_t)(nla * nlb * nlc * nld);
-               const uint32_t row_idx = (uint32_t)Q.rows.size();
-               Q.rows.push_back(r);
-               Q.prm_ip.insert(Q.prm_ip.end(), prm_tmp_.begin(), prm_tmp_.end());
-               Q.prm_row.insert(Q.prm_row.end(), prm_tmp_.size(), row_idx);
-            }
+      g.np4 = (uint32_t)(S[0]->np - 1) | (uint32_t)(S[1]->np - 1) << 4 | (uint32_t)(S[2]->np - 1) << 8 |
+              (uint32_t)(S[3]->np - 1) << 12;
+      g.set_ij = (uint32_t)i | (uint32_t)j << 16;
+      g.set_kl = (uint32_t)k | (uint32_t)l << 16;
+      Q.groups.push_back(g);
+      pend_bytes_ += sizeof(GroupRec) + sizeof(KsGroup) * (size_t)types_[type].plan.nclass;
+   } else if (cur_sets_ != std::array<int, 4>{i, j, k, l}) {
+      fatal("add_shell: rows of a different set quartet before libgint_add_set", i);
+   }
+   if (Q.prm_ip.size() + prm_tmp_.size() > 0xffffffffull) fatal("more than 2^32 primitive quartets of one type", 0);
+   RowRec r;
+   r.prm_off = (uint32_t)Q.prm_ip.size();
+   r.n_prm = (uint32_t)prm_tmp_.size();
+   r.group = (uint32_t)Q.groups.size() - 1;
+   r.n12 = (uint32_t)n1 | (uint32_t)n2 << 16;
+   const uint32_t row_idx = (uint32_t)Q.rows.size();
+   Q.rows.push_back(r);
+   Q.prm_ip.insert(Q.prm_ip.end(), prm_tmp_.begin(), prm_tmp_.end());
+   Q.prm_row.insert(Q.prm_row.end(), prm_tmp_.size(), row_idx);
+   pend_bytes_ += sizeof(RowRec) + 8 * prm_tmp_.size();
    prm_tmp_.clear();
 }
 
@@ -335,18 +413,8 @@ void Engine::add_qrt(int la, int lb, int lc, int ld, int nla, int nlb, int nlc, 
 {
    if (la < 0 || la > MAXL || lb < 0 || lb > MAXL || lc < 0 || lc > MAXL || ld < 0 || ld > MAXL)
       fatal("add_qrt: l must be 0..3", la);
-   const int c = class_index(la, lb, lc, ld);
-   ClassQueue &Q = q_[c];
-   if (!Q.touched) {
-      Q.touched = true;
-      Q.set_row0 = Q.rows.size();
-      Q.set_prm0 = Q.prm_ip.size();
-      Q.set_ks0 = Q.ks.size();
-      Q.set_nq = 0;
-      Q.set_eslots0 = Q.e_slots;
-      set_classes_.push_back(c);
-   }
-   Q.set_nq += (uint32_t)(nla * nlb * nlc * nld);
+   if (nla < 1 || nlb < 1 || nlc < 1 || nld < 1 || nla > MAXNL || nlb > MAXNL || nlc > MAXNL || nld > MAXNL)
+      fatal("add_qrt: 1..4 contractions per shell", nla);
 }
 
 // symmetry factor codes, reference src/libGint.cpp:262-267
@@ -363,23 +431,16 @@ void Engine::add_qrtt(double symm_fac, int la, int lb, int lc, int ld, int inla,
                       int ld_ac, int ld_ad, int ld_bc, int ld_bd, unsigned off_ac, unsigned off_ad, unsigned off_bc,
                       unsigned off_bd, bool Tac, bool Tad, bool Tbc, bool Tbd)
 {
+   // the position of a call among its class's calls, not inla..inld, selects
+   // its contraction (reference src/libGint.cpp:250-280)
    (void)inla; (void)inlb; (void)inlc; (void)inld;
    if (la < 0 || la > MAXL || lb < 0 || lb > MAXL || lc < 0 || lc > MAXL || ld < 0 || ld > MAXL)
       fatal("add_qrtt: l must be 0..3", la);
    if (ld_ac < 0 || ld_ad < 0 || ld_bc < 0 || ld_bd < 0 || ld_ac > 0xffff || ld_ad > 0xffff || ld_bc > 0xffff ||
        ld_bd > 0xffff)
       fatal("add_qrtt: leading dimension out of 0..65535", ld_ac);
-   const int c = class_index(la, lb, lc, ld);
-   ClassQueue &Q = q_[c];
-   if (!Q.touched) {
-      Q.touched = true;
-      Q.set_row0 = Q.rows.size();
-      Q.set_prm0 = Q.prm_ip.size();
-      Q.set_ks0 = Q.ks.size();
-      Q.set_nq = 0;
-      Q.set_eslots0 = Q.e_slots;
-      set_classes_.push_back(c);
-   }
+   const int c = class_code(la, lb, lc, ld);
+   if (pend_[c].empty()) touched_.push_back(c);
    KsRec e;
    e.off[0] = off_ac;
    e.off[1] = off_ad;
@@ -388,65 +449,171 @@ void Engine::add_qrtt(double symm_fac, int la, int lb, int lc, int ld, int inla,
    e.ld01 = (uint32_t)ld_ac | (uint32_t)ld_ad << 16;
    e.ld23 = (uint32_t)ld_bc | (uint32_t)ld_bd << 16;
    e.flags = (uint32_t)Tac | (uint32_t)Tad << 1 | (uint32_t)Tbc << 2 | (uint32_t)Tbd << 3 | symm_code(symm_fac) << 4;
-   e.row0 = e.row1 = 0;
-   e.l = 0;
-   e.pad[0] = e.pad[1] = 0;
-   Q.ks.push_back(e);
+   e.pad = 0;
+   pend_[c].push_back(e);
+   n_q_++;
+}
+
+// The add_qrtt calls of one (set quartet, class) as a KsGroup: offsets
+// affine in (inla, inlb, inlc, inld) (the contraction order of the calls,
+// row-major) with shared leading dimensions and flags, else an explicit list.
+KsGroup Engine::compress(std::vector<KsRec> &calls, const TypePlan &tp, int k, std::vector<KsRec> &ksx)
+{
+   KsGroup g;
+   std::memset(&g, 0, sizeof g);
+   const size_t n = calls.size();
+   if (n == 0) return g;
+   const int sh = tp.cls_sh[k];
+   const int nla = tp.nl[0][sh & 15], nlb = tp.nl[1][(sh >> 4) & 15], nlc = tp.nl[2][(sh >> 8) & 15],
+             nld = tp.nl[3][(sh >> 12) & 15];
+   const size_t nq = (size_t)nla * nlb * nlc * nld;
+   if (n > nq) fatal("add_qrtt: more calls than contraction combinations of the class", (long long)n);
+   // highest element of P / K the calls reach (checked against the matrix
+   // size before any kernel reads or updates them)
+   {
+      const int code = tp.cls_code[k];
+      const uint64_t ns[4] = {(uint64_t)nsph(code / 64), (uint64_t)nsph((code / 16) % 4), (uint64_t)nsph((code / 4) % 4),
+                              (uint64_t)nsph(code % 4)};
+      const int rc[4][2] = {{0, 2}, {0, 3}, {1, 2}, {1, 3}};   // ac ad bc bd: row / column centre
+      for (const KsRec &c : calls)
+         for (int q = 0; q < 4; q++) {
+            const uint64_t ld = q < 2 ? (q == 0 ? c.ld01 & 0xffff : c.ld01 >> 16) : (q == 2 ? c.ld23 & 0xffff : c.ld23 >> 16);
+            const uint64_t nr = ns[rc[q][0]], nc = ns[rc[q][1]];
+            const uint64_t top = c.off[q] + ((c.flags >> q) & 1 ? (nr - 1) * ld + (nc - 1) : (nc - 1) * ld + (nr - 1));
+            if (top > max_index_) max_index_ = top;
+         }
+   }
+   const KsRec &c0 = calls[0];
+   for (int q = 0; q < 4; q++) g.base[q] = c0.off[q];
+   g.ld01 = c0.ld01;
+   g.ld23 = c0.ld23;
+   long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+   auto diff = [&](size_t l, int q) { return (long long)calls[l].off[q] - (long long)c0.off[q]; };
+   if (nld > 1 && n > 1) { st[3] = diff(1, 1); st[7] = diff(1, 3); }
+   if (nlc > 1 && n > (size_t)nld) { st[1] = diff(nld, 0); st[5] = diff(nld, 2); }
+   if (nlb > 1 && n > (size_t)(nlc * nld)) { st[4] = diff(nlc * nld, 2); st[6] = diff(nlc * nld, 3); }
+   if (nla > 1 && n > (size_t)(nlb * nlc * nld)) { st[0] = diff(nlb * nlc * nld, 0); st[2] = diff(nlb * nlc * nld, 1); }
+   bool ok = true;
+   for (long long s : st)
+      if (s < INT32_MIN || s > INT32_MAX) ok = false;
+   for (size_t l = 0; ok && l < n; l++) {
+      const long long ia = (long long)(l / (nlb * nlc * nld)), ib = (long long)((l / (nlc * nld)) % nlb);
+      const long long ic = (long long)((l / nld) % nlc), id = (long long)(l % nld);
+      const KsRec &c = calls[l];
+      ok = c.ld01 == c0.ld01 && c.ld23 == c0.ld23 && c.flags == c0.flags &&
+           (long long)c.off[0] == (long long)c0.off[0] + ia * st[0] + ic * st[1] &&
+           (long long)c.off[1] == (long long)c0.off[1] + ia * st[2] + id * st[3] &&
+           (long long)c.off[2] == (long long)c0.off[2] + ib * st[4] + ic * st[5] &&
+           (long long)c.off[3] == (long long)c0.off[3] + ib * st[6] + id * st[7];
+   }
+   if (ok) {
+      for (int q = 0; q < 8; q++) g.step[q] = (int32_t)st[q];
+      g.flags = c0.flags | (uint32_t)n << 16;
+   } else {
+      g.flags = KS_EXPLICIT | (uint32_t)n << 16;
+      if (ksx.size() + n > 0xffffffffull) fatal("too many explicit add_qrtt records", 0);
+      g.expl = (uint32_t)ksx.size();
+      ksx.insert(ksx.end(), calls.begin(), calls.end());
+      pend_bytes_ += sizeof(KsRec) * n;
+   }
+   return g;
 }
 
 void Engine::add_set()
 {
-   for (int c : set_classes_) {
-      ClassQueue &Q = q_[c];
-      const uint32_t base = Q.n_q;
-      // attach each quartet of the set to the rows that produce it
-      const size_t r0 = Q.set_row0, r1 = Q.rows.size();
-      for (size_t e = Q.set_ks0; e < Q.ks.size(); e++) {
-         KsRec &k = Q.ks[e];
-         const uint32_t q = base + (uint32_t)(e - Q.set_ks0);
-         k.row0 = k.row1 = (uint32_t)r1;
-         k.l = 0;
-         for (size_t r = r0; r < r1; r++) {
-            const RowRec &row = Q.rows[r];
-            const uint32_t nlabcd = (row.nlnp & 15) * ((row.nlnp >> 4) & 15) * ((row.nlnp >> 8) & 15) * ((row.nlnp >> 12) & 15);
-            if (q >= row.q_off && q < row.q_off + nlabcd) {
-               size_t r2 = r;
-               while (r2 < r1 && Q.rows[r2].q_off == row.q_off) r2++;
-               k.row0 = (uint32_t)r;
-               k.row1 = (uint32_t)r2;
-               k.l = q - row.q_off;
-               break;
-            }
-         }
-         if (q >= base + Q.set_nq) k.row0 = k.row1 = (uint32_t)r1;   // no integrals declared for it
-      }
-      Q.n_q += Q.set_nq;
-      Q.touched = false;
-
-      // memory of the pending batch of this class
-      const ClassPlan cp = make_plan(c / 64, (c / 16) % 4, (c / 4) % 4, c % 4);
-      const size_t idx_bytes = Q.rows.size() * sizeof(RowRec) + Q.prm_ip.size() * 8 + Q.ks.size() * sizeof(KsRec);
-      const size_t eco_bytes = (size_t)Q.e_slots * cp.ncp * sizeof(double);
-      const size_t min_ac0 = (size_t)64 * cp.ncp * sizeof(double);
-      if (idx_bytes + eco_bytes + min_ac0 > budget_bytes_ / 2) {
-         const size_t set_idx = (Q.rows.size() - r0) * sizeof(RowRec) + (Q.prm_ip.size() - Q.set_prm0) * 8 +
-                                (Q.ks.size() - Q.set_ks0) * sizeof(KsRec);
-         const size_t set_eco = (size_t)(Q.e_slots - Q.set_eslots0) * cp.ncp * sizeof(double);
-         if (r0 > 0 && set_idx + set_eco + min_ac0 <= budget_bytes_ / 2) {
-            dispatch_class(c, r0, Q.set_prm0, Q.set_ks0);   // earlier sets only
-         } else {
-            dispatch_class(c, Q.rows.size(), Q.prm_ip.size(), Q.ks.size());
-         }
-      }
+   if (cur_type_ >= 0) {
+      TypeQueue &Q = q_[cur_type_];
+      const TypePlan &tp = types_[cur_type_].plan;
+      Q.groups.back().row1 = (uint32_t)Q.rows.size();
+      for (int k = 0; k < tp.nclass; k++) Q.ksg.push_back(compress(pend_[tp.cls_code[k]], tp, k, Q.ksx));
    }
-   set_classes_.clear();
+   // calls of classes without integrals in this set quartet add nothing
+   for (int c : touched_) pend_[c].clear();
+   touched_.clear();
+   cur_type_ = -1;
+   if (!recording_ && pend_bytes_ > budget_bytes_ / 2) dispatch_all();
 }
 
 void Engine::dispatch_all()
 {
-   if (!set_classes_.empty()) fatal("dispatch with an open set: call add_set first", (int)set_classes_.size());
-   for (int c = 0; c < NCLASS; c++)
-      if (!q_[c].ks.empty() || !q_[c].rows.empty()) dispatch_class(c, q_[c].rows.size(), q_[c].prm_ip.size(), q_[c].ks.size());
+   if (cur_type_ >= 0 || !touched_.empty()) fatal("dispatch with an open set: call add_set first", cur_type_);
+   bool any = false;
+   for (auto &Q : q_) any = any || !Q.empty();
+   if (!any || recording_) return;
+   if (!d_P_[0]) fatal("integrals dispatched before set_P", 0);
+   synchronize();   // the previous batch no longer reads the device arrays
+   upload(q_, stream_dev_, false);
+   for (auto &Q : q_) {
+      Q.rows.clear();
+      Q.prm_ip.clear();
+      Q.prm_row.clear();
+      Q.groups.clear();
+      Q.ksg.clear();
+      Q.ksx.clear();
+   }
+   pend_bytes_ = 0;
+   const uint64_t mi = max_index_;
+   max_index_ = 0;
+   execute(stream_dev_, mi);
+}
+
+// ---------------------------------------------------------------------------
+// plan mode: the recorded work stays on the device
+// ---------------------------------------------------------------------------
+void Engine::plan_begin()
+{
+   dispatch_all();
+   recording_ = true;
+}
+
+void Engine::plan_end()
+{
+   if (!recording_) fatal("plan_end without plan_begin", 0);
+   if (cur_type_ >= 0) fatal("plan_end with an open set: call add_set first", cur_type_);
+   recording_ = false;
+   synchronize();
+   for (auto &d : plan_dev_) d.release();
+   plan_dev_.clear();
+   upload(q_, plan_dev_, true);
+   pend_bytes_ = 0;
+   plan_max_index_ = max_index_;
+   max_index_ = 0;
+}
+
+void Engine::plan_execute()
+{
+   if (recording_) fatal("plan_execute while recording", 0);
+   if (!d_P_[0]) fatal("plan_execute before set_P", 0);
+   dispatch_all();
+   execute(plan_dev_, plan_max_index_);
+}
+
+void Engine::plan_clear()
+{
+   synchronize();
+   for (auto &d : plan_dev_) d.release();
+   plan_dev_.clear();
+}
+
+void Engine::plan_info(uint64_t out[6])
+{
+   uint64_t bytes = 0, chunks = 0, groups = 0, rows = 0, prm = 0, nt = 0;
+   for (auto &d : plan_dev_) {
+      if (d.n_groups == 0) continue;
+      nt++;
+      bytes += d.n_rows * sizeof(RowRec) + d.n_groups * sizeof(GroupRec) + d.n_prm * 8 +
+               d.n_groups * (uint64_t)types_[d.type].plan.nclass * sizeof(KsGroup) + d.n_ksx * sizeof(KsRec);
+      chunks += d.chunks.size();
+      groups += d.n_groups;
+      rows += d.n_rows;
+      prm += d.n_prm;
+   }
+   out[0] = bytes;
+   out[1] = nt;
+   out[2] = chunks;
+   out[3] = groups;
+   out[4] = rows;
+   out[5] = prm;
 }
 
 // ---------------------------------------------------------------------------
@@ -498,78 +665,164 @@ void Engine::upload_data()
    data_version_dev_ = store_.version();
 }
 
-void Engine::ensure_arena()
+template <class T>
+static void dev_put(T *&ptr, size_t &cap, const std::vector<T> &v)
 {
-   if (d_arena_ && arena_bytes_ == budget_bytes_) return;
-   if (d_arena_) {
-      synchronize();
-      HIPCHECK(hipFree(d_arena_));
-      HIPCHECK(hipHostFree(h_stage_));
+   if (v.size() > cap || !ptr) {
+      if (ptr) HIPCHECK(hipFree(ptr));
+      const size_t c = std::max<size_t>(v.size() + v.size() / 2, 16);
+      HIPCHECK(hipMalloc((void **)&ptr, sizeof(T) * c));
+      cap = c;
    }
-   arena_bytes_ = budget_bytes_;
-   HIPCHECK(hipMalloc((void **)&d_arena_, arena_bytes_));
-   stage_bytes_ = arena_bytes_ / 2;
-   HIPCHECK(hipHostMalloc((void **)&h_stage_, stage_bytes_, hipHostMallocDefault));
+   if (!v.empty()) HIPCHECK(hipMemcpy(ptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+}
+
+// Copy the queues to the device and split each type's work into chunks
+// whose primitive blocks and contracted rows fit the work buffer.
+void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool release_host)
+{
+   const size_t work = budget_bytes_ / 2;
+   if (devs.size() < qs.size()) devs.resize(qs.size());
+   for (size_t ty = 0; ty < qs.size(); ty++) {
+      TypeQueue &Q = qs[ty];
+      TypeDev &D = devs[ty];
+      D.type = (int)ty;
+      D.chunks.clear();
+      D.n_groups = D.n_rows = D.n_prm = D.n_ksx = 0;
+      if (Q.empty()) continue;
+      TypeInfo &ti = types_[ty];
+      const TypePlan &tp = ti.plan;
+      if (!ti.d_plan) {
+         HIPCHECK(hipMalloc((void **)&ti.d_plan, sizeof(TypePlan)));
+         HIPCHECK(hipMemcpy(ti.d_plan, &tp, sizeof(TypePlan), hipMemcpyHostToDevice));
+      }
+      dev_put(D.rows, D.cap[0], Q.rows);
+      dev_put(D.groups, D.cap[1], Q.groups);
+      dev_put(D.prm_ip, D.cap[2], Q.prm_ip);
+      dev_put(D.prm_row, D.cap[3], Q.prm_row);
+      dev_put(D.ksg, D.cap[4], Q.ksg);
+      dev_put(D.ksx, D.cap[5], Q.ksx);
+      D.n_groups = Q.groups.size();
+      D.n_rows = Q.rows.size();
+      D.n_prm = Q.prm_ip.size();
+      D.n_ksx = Q.ksx.size();
+      D.class_used.assign(tp.nclass, 0);
+      for (size_t gi = 0; gi < D.n_groups; gi++)
+         for (int k = 0; k < tp.nclass; k++)
+            if (ks_count(Q.ksg[gi * tp.nclass + k].flags)) D.class_used[k] = 1;
+      // chunks of whole groups
+      Chunk c{};
+      size_t bytes = 0;
+      const size_t row_b = sizeof(double) * (size_t)tp.ebytes;
+      for (size_t gi = 0; gi < D.n_groups; gi++) {
+         const GroupRec &g = Q.groups[gi];
+         size_t gp = 0;
+         for (uint32_t r = g.row0; r < g.row1; r++) gp += Q.rows[r].n_prm;
+         const size_t gb = gp * sizeof(double) * (size_t)tp.ncp + (size_t)(g.row1 - g.row0) * row_b;
+         if (gb > work) fatal("one set quartet needs more device memory than set_max_mem allows (MiB)", (long long)(gb >> 20));
+         if (bytes + gb > work && c.g1 > c.g0) {
+            D.chunks.push_back(c);
+            c = Chunk{};
+            c.g0 = c.g1 = (uint32_t)gi;
+            c.r0 = c.r1 = g.row0;
+            c.p0 = c.p1 = Q.rows[g.row0].prm_off;
+            bytes = 0;
+         }
+         if (c.g1 == c.g0) {
+            c.g0 = (uint32_t)gi;
+            c.r0 = g.row0;
+            c.p0 = Q.rows[g.row0].prm_off;
+         }
+         c.g1 = (uint32_t)gi + 1;
+         c.r1 = g.row1;
+         c.p1 = (uint32_t)(Q.rows[g.row0].prm_off + gp);
+         bytes += gb;
+         // algorithmic FLOPs of the contraction and of the quartet kernel
+         c.eco_flops += 2.0 * (double)gp * tp.ebytes;
+         for (int k = 0; k < tp.nclass; k++) {
+            const double nq = ks_count(Q.ksg[gi * tp.nclass + k].flags);
+            const ClassPlan &cp = ti.cplans[k];
+            c.k3_flops += nq * ((double)(g.row1 - g.row0) * cp.hrr_flops + cp.sph_flops + 9.0 * cp.Ns);
+         }
+      }
+      if (c.g1 > c.g0) D.chunks.push_back(c);
+      n_rows_ += D.n_rows;
+      if (release_host) Q.release();
+   }
+}
+
+void Engine::ensure_work(size_t ncp_max, size_t max_prm_chunk)
+{
+   const size_t want = budget_bytes_ / 2;
+   if (!d_work_ || work_bytes_ != want) {
+      synchronize();
+      if (d_work_) HIPCHECK(hipFree(d_work_));
+      HIPCHECK(hipMalloc((void **)&d_work_, want));
+      work_bytes_ = want;
+   }
    if (!d_stats_) {
       HIPCHECK(hipMalloc((void **)&d_stats_, 4 * sizeof(unsigned long long)));
       HIPCHECK(hipMemset(d_stats_, 0, 4 * sizeof(unsigned long long)));
+      HIPCHECK(hipMalloc((void **)&d_fixcnt_, 2 * sizeof(uint32_t)));
+      HIPCHECK(hipMemset(d_fixcnt_, 0, 2 * sizeof(uint32_t)));
    }
-   for (auto &e : ev_)
-      if (!e) HIPCHECK(hipEventCreate((hipEvent_t *)&e));
+   const size_t Ng = neighs_.size() / 3;
+   const size_t cap = (debug_ & 1) ? std::max<size_t>(4096, max_prm_chunk * Ng) : 4096;
+   if (cap > fix_cap_) {
+      synchronize();
+      if (d_fix_) HIPCHECK(hipFree(d_fix_));
+      HIPCHECK(hipMalloc((void **)&d_fix_, 2 * sizeof(uint32_t) * cap));
+      fix_cap_ = cap;
+   }
+   const size_t ws = fix_cap_ * (ncp_max + 16);
+   if (ws > fix_ws_cap_) {
+      synchronize();
+      if (d_fix_ws_) HIPCHECK(hipFree(d_fix_ws_));
+      HIPCHECK(hipMalloc((void **)&d_fix_ws_, sizeof(double) * ws));
+      fix_ws_cap_ = ws;
+   }
 }
 
-static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-void Engine::dispatch_class(int c, size_t nrows, size_t nprm, size_t nks)
+void Engine::ev_begin(int cat)
 {
-   ClassQueue &Q = q_[c];
-   if (nrows == 0 && nks == 0) return;
-   if (!d_P_[0]) fatal("integrals dispatched before set_P", c);
-   ensure_arena();
-   ensure_tables();
-   upload_data();
-   hipStream_t st = (hipStream_t)stream_;
-   const ClassPlan cp = make_plan(c / 64, (c / 16) % 4, (c / 4) % 4, c % 4);
-
-   // quartets and e-slots of the prefix
-   uint32_t nq = 0, eslots = 0;
-   if (nrows < Q.rows.size()) {
-      nq = Q.rows[nrows].q_off;
-      eslots = Q.rows[nrows].e_off;
-   } else {
-      nq = Q.n_q;
-      eslots = Q.e_slots;
+   (void)cat;
+   if (!timing_on_) return;
+   if (ev_used_ + 2 > ev_pool_.size()) {
+      hipEvent_t e0, e1;
+      HIPCHECK(hipEventCreate(&e0));
+      HIPCHECK(hipEventCreate(&e1));
+      ev_pool_.push_back(e0);
+      ev_pool_.push_back(e1);
    }
-   // index data -> pinned staging -> arena
-   const size_t b_rows = align256(nrows * sizeof(RowRec));
-   const size_t b_prow = align256(nprm * 4), b_pip = align256(nprm * 4);
-   const size_t b_ks = align256(nks * sizeof(KsRec));
-   const size_t idx_bytes = b_rows + b_prow + b_pip + b_ks;
-   const size_t eco_bytes = align256((size_t)eslots * cp.ncp * sizeof(double));
-   if (idx_bytes > stage_bytes_ || idx_bytes + eco_bytes + (size_t)64 * cp.ncp * 8 > arena_bytes_)
-      fatal("one set quartet needs more device memory than set_max_mem allows", (int)((idx_bytes + eco_bytes) >> 20));
-   HIPCHECK(hipStreamSynchronize(st));   // staging and arena are free again
-   char *hs = h_stage_;
-   std::memcpy(hs, Q.rows.data(), nrows * sizeof(RowRec));
-   std::memcpy(hs + b_rows, Q.prm_row.data(), nprm * 4);
-   std::memcpy(hs + b_rows + b_prow, Q.prm_ip.data(), nprm * 4);
-   std::memcpy(hs + b_rows + b_prow + b_pip, Q.ks.data(), nks * sizeof(KsRec));
-   HIPCHECK(hipMemcpyAsync(d_arena_, h_stage_, idx_bytes, hipMemcpyHostToDevice, st));
+   ev_open_ = ev_pool_[ev_used_];
+   HIPCHECK(hipEventRecord((hipEvent_t)ev_open_, (hipStream_t)stream_));
+}
 
-   BatchDev b;
-   b.rows = (const RowRec *)d_arena_;
-   b.prm_row = (const uint32_t *)(d_arena_ + b_rows);
-   b.prm_ip = (const uint32_t *)(d_arena_ + b_rows + b_prow);
-   b.ks = (const KsRec *)(d_arena_ + b_rows + b_prow + b_pip);
-   b.data = d_data_;
-   b.eco = (double *)(d_arena_ + idx_bytes);
-   b.ac0 = (double *)(d_arena_ + idx_bytes + eco_bytes);
-   b.n_rows = (uint32_t)nrows;
-   b.n_prm = (uint32_t)nprm;
-   b.n_q = (uint32_t)nks;
-   b.stats = stats_on_ ? d_stats_ : nullptr;
-   const size_t ac0_cap = (arena_bytes_ - idx_bytes - eco_bytes) / (sizeof(double) * cp.ncp);
+void Engine::ev_end(int cat)
+{
+   if (!timing_on_) return;
+   void *e1 = ev_pool_[ev_used_ + 1];
+   HIPCHECK(hipEventRecord((hipEvent_t)e1, (hipStream_t)stream_));
+   ev_live_.push_back(EvRec{cat, ev_open_, e1, 0.0});
+   ev_used_ += 2;
+}
 
+void Engine::harvest_events()
+{
+   for (const EvRec &r : ev_live_) {
+      float ms = 0.f;
+      HIPCHECK(hipEventElapsedTime(&ms, (hipEvent_t)r.e0, (hipEvent_t)r.e1));
+      timing_.ms[r.cat] += ms;
+      timing_.launches[r.cat]++;
+   }
+   ev_live_.clear();
+   ev_used_ = 0;
+}
+
+void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
+{
+   const TypePlan &tp = ti.plan;
+   hipStream_t st = (hipStream_t)stream_;
    DevTables t;
    t.C0 = d_C0_;
    t.ld_C0 = ld_C0_;
@@ -583,44 +836,30 @@ void Engine::dispatch_class(int c, size_t nrows, size_t nprm, size_t nks)
    t.truncated = truncated_ ? 1 : 0;
    t.R_cut = R_cut_;
 
-   const size_t need = scratch_doubles(cp);
-   if (need > scratch_cap_) {
-      if (d_scratch_) HIPCHECK(hipFree(d_scratch_));
-      HIPCHECK(hipMalloc((void **)&d_scratch_, need * sizeof(double)));
-      scratch_cap_ = need;
-   }
-   Scratch sc{d_scratch_, scratch_cap_};
+   BatchDev b;
+   b.tp = ti.d_plan;
+   b.rows = td.rows;
+   b.groups = td.groups;
+   b.prm_row = td.prm_row;
+   b.prm_ip = td.prm_ip;
+   b.ksg = td.ksg;
+   b.ksx = td.ksx;
+   b.data = d_data_;
+   b.ac0 = d_work_;
+   b.eco = d_work_ + (size_t)(c.p1 - c.p0) * tp.ncp;
+   b.p0 = c.p0;
+   b.p1 = c.p1;
+   b.r0 = c.r0;
+   b.r1 = c.r1;
+   b.g0 = c.g0;
+   b.g1 = c.g1;
+   b.stats = stats_on_ ? d_stats_ : nullptr;
+   b.fix = d_fix_;
+   b.fix_count = d_fixcnt_;
+   b.fix_cap = (uint32_t)fix_cap_;
+   b.force_fix = debug_ & 1;
+   b.fix_ws = d_fix_ws_;
 
-   if (eco_bytes) HIPCHECK(hipMemsetAsync(b.eco, 0, eco_bytes, st));
-   const bool tm = timing_on_;
-   // primitives in chunks that fit the arena; the contraction accumulates
-   for (size_t p0 = 0; p0 < nprm; p0 += ac0_cap) {
-      const size_t p1 = std::min(nprm, p0 + ac0_cap);
-      BatchDev bc = b;
-      bc.prm_row = b.prm_row + p0;
-      bc.prm_ip = b.prm_ip + p0;
-      bc.n_prm = (uint32_t)(p1 - p0);
-      if (tm) HIPCHECK(hipEventRecord((hipEvent_t)ev_[0], st));
-      int rc = launch_fm_vrr(cp, t, bc, sc, st);
-      if (rc) fatal("fm_vrr launch", rc);
-      if (tm) HIPCHECK(hipEventRecord((hipEvent_t)ev_[1], st));
-      // rows whose primitives intersect [p0, p1)
-      const uint32_t first_row = Q.prm_row[p0];
-      const uint32_t last_row = Q.prm_row[p1 - 1];
-      rc = launch_eco_range(cp, b, first_row, last_row + 1, (uint32_t)p0, (uint32_t)p1, st);
-      if (rc) fatal("eco launch", rc);
-      if (tm) {
-         HIPCHECK(hipEventRecord((hipEvent_t)ev_[2], st));
-         HIPCHECK(hipEventSynchronize((hipEvent_t)ev_[2]));
-         float ms0 = 0, ms1 = 0;
-         HIPCHECK(hipEventElapsedTime(&ms0, (hipEvent_t)ev_[0], (hipEvent_t)ev_[1]));
-         HIPCHECK(hipEventElapsedTime(&ms1, (hipEvent_t)ev_[1], (hipEvent_t)ev_[2]));
-         timing_.ms[0] += ms0;
-         timing_.ms[1] += ms1;
-         timing_.launches[0]++;
-         timing_.launches[1]++;
-      }
-   }
    KTargets kt;
    kt.nspin = nspin_;
    kt.hf_fac = hf_fac_;
@@ -628,68 +867,74 @@ void Engine::dispatch_class(int c, size_t nrows, size_t nprm, size_t nks)
       kt.P[s] = d_P_[s];
       kt.K[s] = d_K_[s];
    }
-   if (tm) HIPCHECK(hipEventRecord((hipEvent_t)ev_[3], st));
-   int rc = launch_hrr_sph_ks(cp, t, b, kt, sc, st);
-   if (rc) fatal("hrr_sph_ks launch", rc);
-   if (tm) {
-      HIPCHECK(hipEventRecord((hipEvent_t)ev_[4], st));
-      HIPCHECK(hipEventSynchronize((hipEvent_t)ev_[4]));
-      float ms = 0;
-      HIPCHECK(hipEventElapsedTime(&ms, (hipEvent_t)ev_[3], (hipEvent_t)ev_[4]));
-      timing_.ms[2] += ms;
-      timing_.launches[2]++;
-   }
-   {  // algorithmic FLOPs of the contraction and of the quartet kernel
-      double eco = 0, qk = 0;
-      for (size_t r = 0; r < nrows; r++) {
-         const RowRec &row = Q.rows[r];
-         const double nlabcd = (double)((row.nlnp & 15) * ((row.nlnp >> 4) & 15) * ((row.nlnp >> 8) & 15) * ((row.nlnp >> 12) & 15));
-         eco += 2.0 * nlabcd * cp.ncp * row.n_prm;
-         qk += nlabcd * cp.hrr_flops;
-      }
-      qk += (double)nks * (cp.sph_flops + 9.0 * cp.Ns);
-      timing_.flops[1] += eco;
-      timing_.flops[2] += qk;
-   }
-   n_slots_ += (uint64_t)nprm * (uint64_t)t.Ng;
-   n_prm_ += nprm;
-   n_q_ += nks;
-   n_rows_ += nrows;
 
-   // drop the dispatched prefix, rebase the rest
-   Q.rows.erase(Q.rows.begin(), Q.rows.begin() + (long)nrows);
-   Q.prm_row.erase(Q.prm_row.begin(), Q.prm_row.begin() + (long)nprm);
-   Q.prm_ip.erase(Q.prm_ip.begin(), Q.prm_ip.begin() + (long)nprm);
-   Q.ks.erase(Q.ks.begin(), Q.ks.begin() + (long)nks);
-   for (RowRec &r : Q.rows) {
-      r.prm_off -= (uint32_t)nprm;
-      r.q_off -= nq;
-      r.e_off -= eslots;
+   HIPCHECK(hipMemsetAsync(d_fixcnt_, 0, sizeof(uint32_t), st));
+   ev_begin(0);
+   int rc = launch_fm_r(tp, t, b, st);
+   if (rc) fatal("fm_r launch", rc);
+   ev_end(0);
+   ev_begin(1);
+   rc = launch_eco(tp, b, st);
+   if (rc) fatal("eco launch", rc);
+   ev_end(1);
+   ev_begin(3);
+   rc = launch_fixup(tp, t, b, (debug_ & 1) ? (uint32_t)fix_cap_ : 64u, st);
+   if (rc) fatal("fix-up launch", rc);
+   ev_end(3);
+   ev_begin(2);
+   for (int k = 0; k < tp.nclass; k++) {
+      if (!td.class_used[k]) continue;
+      const ClassPlan &cp = ti.cplans[k];
+      rc = launch_hrr_sph_ks(cp, k, tp, t, b, kt, Scratch{d_scratch_, scratch_cap_}, st);
+      if (rc) fatal("hrr_sph_ks launch", rc);
    }
-   for (uint32_t &r : Q.prm_row) r -= (uint32_t)nrows;
-   for (KsRec &k : Q.ks) {
-      k.row0 -= (uint32_t)nrows;
-      k.row1 -= (uint32_t)nrows;
-   }
-   Q.n_q -= nq;
-   Q.e_slots -= eslots;
-   if (Q.touched) {
-      Q.set_row0 -= nrows;
-      Q.set_prm0 -= nprm;
-      Q.set_ks0 -= nks;
-      Q.set_eslots0 -= eslots;
-   }
+   ev_end(2);
+   timing_.flops[1] += c.eco_flops;
+   timing_.flops[2] += c.k3_flops;
+   n_prmc_ += (uint64_t)(c.p1 - c.p0) * tp.nclass;
+   n_slots_ += (uint64_t)(c.p1 - c.p0) * (uint64_t)t.Ng * tp.nclass;
 }
 
-void Engine::enable_stats(bool on) { stats_on_ = on; }
-void Engine::enable_timing(bool on) { timing_on_ = on; }
+void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
+{
+   if (max_index >= FP_size_) fatal("add_qrtt offsets reach beyond the P / K size of set_P", (long long)max_index);
+   ensure_tables();
+   upload_data();
+   size_t ncp_max = 0, max_prm = 0, scr = 0;
+   for (auto &D : devs) {
+      if (D.chunks.empty()) continue;
+      const TypeInfo &ti = types_[D.type];
+      ncp_max = std::max(ncp_max, (size_t)ti.plan.ncp);
+      for (auto &c : D.chunks) max_prm = std::max(max_prm, (size_t)(c.p1 - c.p0));
+      for (auto &cp : ti.cplans) scr = std::max(scr, scratch_doubles(cp));
+   }
+   if (ncp_max == 0) return;
+   ensure_work(ncp_max, max_prm);
+   if (scr > scratch_cap_) {
+      synchronize();
+      if (d_scratch_) HIPCHECK(hipFree(d_scratch_));
+      HIPCHECK(hipMalloc((void **)&d_scratch_, scr * sizeof(double)));
+      scratch_cap_ = scr;
+   }
+   for (auto &D : devs)
+      for (auto &c : D.chunks) run_chunk(types_[D.type], D, c);
+}
 
+// ---------------------------------------------------------------------------
+// counters
+// ---------------------------------------------------------------------------
 void Engine::reset_counters()
 {
    synchronize();
-   n_slots_ = n_prm_ = n_q_ = n_rows_ = 0;
+   n_slots_ = n_prmc_ = n_q_ = n_rows_ = 0;
    timing_ = Timing();
    if (d_stats_) HIPCHECK(hipMemset(d_stats_, 0, 4 * sizeof(unsigned long long)));
+}
+
+Timing Engine::timing()
+{
+   synchronize();
+   return timing_;
 }
 
 void Engine::stats(uint64_t out[8])
@@ -699,7 +944,7 @@ void Engine::stats(uint64_t out[8])
    if (d_stats_) HIPCHECK(hipMemcpy(s, d_stats_, sizeof s, hipMemcpyDeviceToHost));
    out[0] = n_slots_;
    out[1] = s[0];
-   out[2] = n_prm_;
+   out[2] = n_prmc_;
    out[3] = n_q_;
    out[4] = s[1];
    out[5] = n_rows_;

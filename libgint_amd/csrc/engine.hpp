@@ -1,10 +1,21 @@
 // Host engine behind the libgint_* C-ABI (one object per handle, i.e. per
 // OpenMP thread of the caller).  Mirrors the state and call protocol of the
 // reference class libGint (reference src/libGint.h:53-229,
-// src/libGint.cpp), with its own batching and device layout.
+// src/libGint.cpp), with its own batching and device layout:
+//
+//  * work is grouped by quartet type (shell structure of the four sets);
+//    one row per add_shell call keeps its primitive list once for every
+//    class of the set quartet;
+//  * the add_qrtt calls of a (set quartet, class) are compressed to offsets
+//    affine in the contraction indices (or kept as an explicit list);
+//  * pending work is uploaded and run when it exceeds the memory budget or
+//    at dispatch / get_K, or, in plan mode, kept resident on the device and
+//    run again for every new density.
 #pragma once
 #include <stdint.h>
 
+#include <array>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -32,25 +43,51 @@ struct SetInfo {
    int np = 0;
    std::vector<int> l, nl;
    std::vector<uint32_t> K;
+   int sig = -1;   // shell-structure id, -1 until needed
 };
 
-// pending work of one angular-momentum class
-struct ClassQueue {
+// host-side pending work of one quartet type
+struct TypeQueue {
    std::vector<RowRec> rows;
-   std::vector<uint32_t> prm_row, prm_ip;
-   std::vector<KsRec> ks;
-   uint32_t n_q = 0;       // contracted quartets committed
-   uint32_t e_slots = 0;   // sum of nlabcd over rows
-   // current set
-   size_t set_row0 = 0, set_prm0 = 0, set_ks0 = 0;
-   uint32_t set_nq = 0, set_eslots0 = 0;
-   bool touched = false;
+   std::vector<uint32_t> prm_ip, prm_row;
+   std::vector<GroupRec> groups;
+   std::vector<KsGroup> ksg;
+   std::vector<KsRec> ksx;
+   bool empty() const { return groups.empty(); }
+   void release();
+};
+
+struct Chunk {
+   uint32_t g0, g1, r0, r1, p0, p1;
+   double eco_flops, k3_flops;
+};
+
+// one type's work on the device
+struct TypeDev {
+   int type = -1;
+   RowRec *rows = nullptr;
+   GroupRec *groups = nullptr;
+   uint32_t *prm_ip = nullptr, *prm_row = nullptr;
+   KsGroup *ksg = nullptr;
+   KsRec *ksx = nullptr;
+   size_t cap[6] = {0, 0, 0, 0, 0, 0};   // allocated elements of each array
+   size_t n_rows = 0, n_groups = 0, n_prm = 0, n_ksx = 0;
+   std::vector<Chunk> chunks;
+   std::vector<char> class_used;
+   void release();
+};
+
+struct TypeInfo {
+   std::array<int, 4> sig;
+   TypePlan plan;
+   TypePlan *d_plan = nullptr;
+   std::vector<ClassPlan> cplans;
 };
 
 struct Timing {
-   double ms[3] = {0, 0, 0};   // fm_vrr, eco, hrr_sph_ks
-   uint64_t launches[3] = {0, 0, 0};
-   double flops[3] = {0, 0, 0};  // algorithmic FLOPs of eco and hrr_sph_ks (host-counted); [0] from stats
+   double ms[4] = {0, 0, 0, 0};        // fm_r, eco, hrr_sph_ks, fixup
+   uint64_t launches[4] = {0, 0, 0, 0};
+   double flops[4] = {0, 0, 0, 0};     // host-counted algorithmic FLOPs of eco and hrr_sph_ks
 };
 
 class Engine {
@@ -82,20 +119,33 @@ class Engine {
    void synchronize();
 
    // extensions (not in the reference ABI)
-   void enable_stats(bool on);
+   void enable_stats(bool on) { stats_on_ = on; }
    void stats(uint64_t out[8]);
-   void enable_timing(bool on);
-   Timing timing() const { return timing_; }
+   void enable_timing(bool on) { timing_on_ = on; }
+   Timing timing();
    void reset_counters();
+   void set_debug(int flags) { debug_ = flags; }
+   void plan_begin();
+   void plan_end();
+   void plan_execute();
+   void plan_clear();
+   void plan_info(uint64_t out[6]);
    int np_of(int i) const { return i < (int)sets_.size() ? sets_[i].np : 0; }
    const double *exponents_of(int i) const { return &store_.data()[sets_[i].Z]; }
 
  private:
-   void dispatch_class(int c, size_t nrows, size_t nprm, size_t nks);
-   void ensure_arena();
+   int type_of(int i, int j, int k, int l);
+   int set_sig(int i);
+   KsGroup compress(std::vector<KsRec> &calls, const TypePlan &tp, int k, std::vector<KsRec> &ksx);
+   void upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool release_host);
+   void execute(std::vector<TypeDev> &devs, uint64_t max_index);
+   void run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c);
+   void ensure_work(size_t ncp_max, size_t max_prm_chunk);
    void ensure_tables();
    void upload_data();
-   void free_device_matrices();
+   void harvest_events();
+   void ev_begin(int cat);
+   void ev_end(int cat);
    int my_thr_ = 0, nomp_ = 1;
    void *stream_ = nullptr;
 
@@ -112,30 +162,46 @@ class Engine {
    // basis
    DataStore store_;
    std::vector<SetInfo> sets_;
+   std::map<std::pair<std::vector<int>, std::vector<int>>, int> sig_ids_;
+   std::map<std::array<int, 4>, int> type_ids_;
+   std::vector<TypeInfo> types_;
    uint64_t data_version_dev_ = ~0ull;
+   std::array<int, 4> last_sets_ = {-1, -1, -1, -1};
+   int last_type_ = -1;
 
-   // work
+   // ingestion
    std::vector<uint32_t> prm_tmp_;
-   ClassQueue q_[NCLASS];
-   std::vector<int> set_classes_;
+   std::vector<TypeQueue> q_;
+   int cur_type_ = -1;
+   std::array<int, 4> cur_sets_ = {-1, -1, -1, -1};
+   std::vector<KsRec> pend_[NCLASS];
+   std::vector<int> touched_;
+   size_t pend_bytes_ = 0;
+   uint64_t max_index_ = 0;   // highest P / K element the pending calls reach
+   uint64_t plan_max_index_ = 0;
    size_t budget_bytes_ = 0;
+   bool recording_ = false;
+   int debug_ = 0;
 
    // device
+   std::vector<TypeDev> stream_dev_, plan_dev_;
    double *d_data_ = nullptr;
    size_t d_data_cap_ = 0;
    double *d_C0_ = nullptr;
    int *d_grid_ = nullptr;
    double *d_bw_ = nullptr, *d_ftab_ = nullptr, *d_cell_ = nullptr, *d_neighs_ = nullptr;
    size_t d_neighs_cap_ = 0;
-   char *d_arena_ = nullptr;
-   size_t arena_bytes_ = 0;
-   char *h_stage_ = nullptr;
-   size_t stage_bytes_ = 0;
+   double *d_work_ = nullptr;   // ac0 + eco of one chunk
+   size_t work_bytes_ = 0;
+   uint32_t *d_fix_ = nullptr;
+   double *d_fix_ws_ = nullptr;
+   size_t fix_cap_ = 0, fix_ws_cap_ = 0;
    double *d_scratch_ = nullptr;
    size_t scratch_cap_ = 0;
-   unsigned long long *d_stats_ = nullptr;
+   unsigned long long *d_stats_ = nullptr;   // [0..2] kernel statistics, [3] unused
+   uint32_t *d_fixcnt_ = nullptr;             // [0] entries of the chunk, [1] overflow (sticky)
    bool stats_on_ = false;
-   uint64_t n_slots_ = 0, n_prm_ = 0, n_q_ = 0, n_rows_ = 0;
+   uint64_t n_slots_ = 0, n_prmc_ = 0, n_q_ = 0, n_rows_ = 0;
 
    // P and K: shared by the team (reference src/libGint.cpp:441-447,558-562)
    int nspin_ = 0;
@@ -145,9 +211,17 @@ class Engine {
 
    bool timing_on_ = false;
    Timing timing_;
-   void *ev_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+   struct EvRec {
+      int cat;
+      void *e0, *e1;
+      double flops;
+   };
+   std::vector<void *> ev_pool_;
+   std::vector<EvRec> ev_live_;
+   size_t ev_used_ = 0;
+   void *ev_open_ = nullptr;
 };
 
-void fatal(const char *what, int code);
+void fatal(const char *what, long long code);
 
 }  // namespace gint

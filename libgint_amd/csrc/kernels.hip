@@ -1,18 +1,28 @@
-// gfx950 kernels of the K-build, one angular-momentum class per launch.
+// gfx950 kernels of the K-build.  A launch covers one chunk of one quartet
+// type (the shell structure of the four sets of a set quartet).
 //
-//   k_fm_vrr       one wavefront per primitive quartet.  Lanes evaluate
-//                  G_m(R,T) (or F_m) for 64 lattice cells at once, then the
-//                  wavefront runs the Obara-Saika VRR of every surviving cell
-//                  in LDS and sums the (e0|f0) blocks over cells on chip.
-//                  Replaces reference src/compute_Fm.cpp (prepare, Fm, Vm),
-//                  src/compute_VRR.cpp and the cell sum of
-//                  src/compute_ECO.cpp:216-272; nothing per cell touches HBM.
-//   k_eco          one workgroup per row: contraction over primitives,
-//                  reference src/compute_ECO.cpp:12-214.
-//   k_hrr_sph_ks   one wavefront per contracted quartet: HRR of each (n1,n2)
-//                  row, summed on chip (no atomics), cartesian -> spherical,
-//                  exchange update of K.  Reference src/compute_HRR.cpp,
-//                  src/compute_SPH.cpp:58-237, src/compute_KS.cpp:8-117.
+//   k_fm_r         one wavefront per primitive quartet.  Lanes evaluate
+//                  G_m(R,T) (or F_m) of 64 lattice cells at once and decide
+//                  Fm[0] > 1e-12 for every class L of the type; surviving
+//                  cells run the McMurchie-Davidson Hermite recursion
+//                  R^(n)_tuv in LDS and the n = 0 layer is summed over cells
+//                  in registers.  One Hermite -> cartesian transform per
+//                  primitive quartet then gives the cell-summed (e0|f0)
+//                  blocks of every class of the type.  Replaces, for all
+//                  classes at once, reference src/compute_Fm.cpp (prepare,
+//                  Fm, Vm), src/compute_VRR.cpp and the cell sum of
+//                  src/compute_ECO.cpp:216-272.
+//   k_eco          one workgroup per row: contraction over primitives for
+//                  every class, reference src/compute_ECO.cpp:12-214.
+//   k_fm_r<FIX>,   slots whose screening differs between the classes of the
+//   k_fix_eco      type (Fm[0] within rounding of 1e-12 at different L): the
+//                  slot's blocks alone, added to the classes that keep it.
+//   k_hrr_sph_ks   one wavefront per (set quartet, class): HRR of each
+//                  (n1,n2) row summed on chip, cartesian -> spherical, and
+//                  the exchange update of K, reduced over the set quartet's
+//                  contractions before the atomics.  Reference
+//                  src/compute_HRR.cpp, src/compute_SPH.cpp:58-237,
+//                  src/compute_KS.cpp:8-117.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -24,13 +34,15 @@
 namespace gint {
 
 // ---------------------------------------------------------------------------
-// cartesian component tables for shells 0..7, built at compile time
+// cartesian and Hermite index tables, built at compile time
 // ---------------------------------------------------------------------------
+constexpr int NSHELL = 13;  // shells 0..12
+
 struct CartTables {
-   unsigned char comp[8][36][3];   // lx ly lz of component i of shell L
-   unsigned char first[8][36];     // first non-zero direction
-   signed char minus[8][36][3];    // index of i - 1_d in shell L-1, -1 if none
-   signed char plus[8][36][3];     // index of i + 1_d in shell L+1
+   unsigned char comp[NSHELL][91][3];  // lx ly lz of component i of shell L
+   unsigned char first[NSHELL][91];    // first non-zero direction
+   signed char minus[NSHELL][91][3];   // index of i - 1_d in shell L-1, -1 if none
+   signed char plus[7][28][3];         // index of i + 1_d in shell L+1
 };
 
 constexpr int cart_index(int L, int lx, int lz) { return (L - lx) * (L - lx + 1) / 2 + lz; }
@@ -38,7 +50,7 @@ constexpr int cart_index(int L, int lx, int lz) { return (L - lx) * (L - lx + 1)
 constexpr CartTables make_cart_tables()
 {
    CartTables t{};
-   for (int L = 0; L < 8; L++) {
+   for (int L = 0; L < NSHELL; L++) {
       int i = 0;
       for (int lx = L; lx >= 0; lx--)
          for (int ly = L - lx; ly >= 0; ly--) {
@@ -54,9 +66,11 @@ constexpr CartTables make_cart_tables()
                } else {
                   t.minus[L][i][d] = -1;
                }
-               int u[3] = {v[0], v[1], v[2]};
-               u[d]++;
-               t.plus[L][i][d] = (signed char)cart_index(L + 1, u[0], u[2]);
+               if (L < 7) {
+                  int u[3] = {v[0], v[1], v[2]};
+                  u[d]++;
+                  t.plus[L][i][d] = (signed char)cart_index(L + 1, u[0], u[2]);
+               }
             }
             i++;
          }
@@ -64,11 +78,64 @@ constexpr CartTables make_cart_tables()
    return t;
 }
 
+// Hermite triples (t,u,v), ordered by N = t+u+v, then like the cartesian
+// components of shell N; the first nherm(K) entries are those with N <= K.
+struct HermEntry {
+   short m1, m2;          // index of tuv - 1_dir and tuv - 2_dir (-1 if none)
+   unsigned char dir;     // recursion direction: first non-zero of t, u, v
+   unsigned char coef;    // component along dir, minus one
+   unsigned char N, pad;
+};
+struct HermTables {
+   HermEntry e[MAXHERM];
+   unsigned char tuv[MAXHERM][3];
+   short idx[NSHELL][NSHELL][NSHELL];
+};
+
+constexpr HermTables make_herm_tables()
+{
+   HermTables h{};
+   int n = 0;
+   for (int N = 0; N < NSHELL; N++)
+      for (int t = N; t >= 0; t--)
+         for (int u = N - t; u >= 0; u--) {
+            const int v = N - t - u;
+            h.tuv[n][0] = (unsigned char)t;
+            h.tuv[n][1] = (unsigned char)u;
+            h.tuv[n][2] = (unsigned char)v;
+            h.idx[t][u][v] = (short)n;
+            n++;
+         }
+   for (int i = 0; i < n; i++) {
+      const int t = h.tuv[i][0], u = h.tuv[i][1], v = h.tuv[i][2];
+      HermEntry e{};
+      e.N = (unsigned char)(t + u + v);
+      e.m1 = e.m2 = -1;
+      if (t + u + v > 0) {
+         const int d = t > 0 ? 0 : (u > 0 ? 1 : 2);
+         int w[3] = {t, u, v};
+         const int c = w[d];
+         e.dir = (unsigned char)d;
+         e.coef = (unsigned char)(c - 1);
+         w[d]--;
+         e.m1 = h.idx[w[0]][w[1]][w[2]];
+         if (c >= 2) {
+            w[d]--;
+            e.m2 = h.idx[w[0]][w[1]][w[2]];
+         }
+      }
+      h.e[i] = e;
+   }
+   return h;
+}
+
 __constant__ CartTables c_cart = make_cart_tables();
+__constant__ HermTables c_herm = make_herm_tables();
 __constant__ double c_c2s[GINT_C2S_SIZE] = {GINT_C2S_VALUES};
 __constant__ int c_c2s_off[4] = {GINT_C2S_OFF_0, GINT_C2S_OFF_1, GINT_C2S_OFF_2, GINT_C2S_OFF_3};
 
 __device__ __forceinline__ int d_nco(int l) { return (l + 1) * (l + 2) / 2; }
+__device__ __forceinline__ int d_nherm(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
 
 // ---------------------------------------------------------------------------
 // per-primitive quantities, in the order of reference
@@ -77,8 +144,8 @@ __device__ __forceinline__ int d_nco(int l) { return (l + 1) * (l + 2) / 2; }
 // ---------------------------------------------------------------------------
 struct PrimGeo {
    double P[3], Q[3], PA[3], QC[3];
-   double Kfac, inv_z, zab, zcd, rho, Rr;
-   double i2ab, mrab, i2cd, mrcd, i2z;
+   double Kfac, zab, zcd, rho, Rr;
+   double i2ab, i2cd;
 };
 
 __device__ void prim_prepare(const double *Ao, const double *Bo, const double *Co, const double *Do,
@@ -138,67 +205,12 @@ __device__ void prim_prepare(const double *Ao, const double *Bo, const double *C
       g.PA[k] = P[k] - A[k];
       g.QC[k] = Q[k] - Cs[k];
    }
-   g.inv_z = inv_z;
    g.zab = zab;
-   g.zcd = zc + zd;
-   g.rho = zab * g.zcd * inv_z;
+   g.zcd = zcd;
+   g.rho = zab * zcd * inv_z;
    g.Rr = t.R_cut * sqrt(g.rho);
-   const double izab = 1. / zab;
-   g.i2ab = 0.5 * izab;
-   g.mrab = -1. * g.rho * izab * g.i2ab;
-   const double izcd = 1. / g.zcd;
-   g.i2cd = 0.5 * izcd;
-   g.mrcd = -1. * g.rho * izcd * g.i2cd;
-   g.i2z = 0.5 * inv_z;
-}
-
-// Fm[0..L] of one cell, reference src/compute_Fm.cpp:212-236 and
-// src/t_c_g0_n.cpp:67-180.  Fm[0] is evaluated without contraction.
-__device__ void slot_fm(const DevTables &t, int L, const PrimGeo &g, int n3, double *Fm)
-{
-   double T;
-   {
-      GINT_NO_CONTRACT
-      double PQ0 = g.P[0] - g.Q[0] - t.neighs[n3 * 3 + 0];
-      double PQ1 = g.P[1] - g.Q[1] - t.neighs[n3 * 3 + 1];
-      double PQ2 = g.P[2] - g.Q[2] - t.neighs[n3 * 3 + 2];
-      double rpq2 = PQ0 * PQ0 + PQ1 * PQ1 + PQ2 * PQ2;
-      T = g.rho * rpq2;
-   }
-   if (!t.truncated) {
-      gint_boys(L, T, Fm, t.ftable, g.Kfac);
-      return;
-   }
-   int patch;
-   double TG1, TG2;
-   int br = gint_tcg_locate(g.Rr, T, t.grid_lo, t.grid_hi, t.bw, &patch, &TG1, &TG2);
-   if (br == GINT_G_ZERO) {
-      for (int k = 0; k <= L; k++) Fm[k] = 0.0;
-      return;
-   }
-   if (br == GINT_G_BOYS) {
-      gint_boys(L, T, Fm, t.ftable, g.Kfac);
-      return;
-   }
-   double T1[14], T2[14];
-   gint_cheb14(TG1, T1);
-   gint_cheb14(TG2, T2);
-   const double *row = t.C0 + (size_t)t.ld_C0 * patch;
-   {
-      GINT_NO_CONTRACT
-      Fm[0] = gint_cheb_sum(row, T1, T2) * g.Kfac;
-   }
-   for (int k = 1; k <= L; k++) {
-      const double *c = row + k * TCG_NCOEF;
-      double tmp = 0.0;
-      int jl = 0;
-      for (int j = 0; j < 14; j++) {
-         double dot = 0.0;
-         for (int l = 0; l < 14 - j; l++) dot += T1[l] * c[jl++];
-         tmp += dot * T2[j];
-      }
-      Fm[k] = tmp * g.Kfac;
-   }
+   g.i2ab = 0.5 * inv_zab;
+   g.i2cd = 0.5 * inv_zcd;
 }
 
 // order-independent survivor checksum, same function as the oracle's
@@ -209,258 +221,519 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z)
    return z ^ (z >> 31);
 }
 
-// ---------------------------------------------------------------------------
-// VRR of one slot in the workspace, lanes over elements.
-// ws block (a,c) at voff[a][c], layout [m][ia][ic], m = 0..L-a-c.
-// pq: PA WP QC WQ (3 each) i2ab mrab i2cd mrcd i2z
-// ---------------------------------------------------------------------------
-__device__ void vrr_slot(const ClassPlan &cp, double *ws, const double *Fm, const double *pq, int lane, int nlane)
+__device__ __forceinline__ unsigned long long slot_hash(uint32_t set_ij, uint32_t set_kl, int n1, int n2, uint32_t ip,
+                                                        int n3)
 {
-   const int L = cp.L;
-   for (int m = lane; m <= L; m += nlane) ws[cp.voff[0][0] + m] = Fm[m];
-   __syncthreads();
-   const double i2ab = pq[12], mrab = pq[13], i2cd = pq[14], mrcd = pq[15], i2z = pq[16];
-   for (int a = 1; a <= cp.LA; a++) {
-      const int na = d_nco(a), na1 = d_nco(a - 1), na2 = a >= 2 ? d_nco(a - 2) : 1;
-      const int n = (L - a + 1) * na;
-      const double *V1 = ws + cp.voff[a - 1][0];
-      const double *V2 = ws + (a >= 2 ? cp.voff[a - 2][0] : 0);
-      double *V = ws + cp.voff[a][0];
-      for (int idx = lane; idx < n; idx += nlane) {
-         const int m = idx / na, ia = idx - m * na;
-         const int d = c_cart.first[a][ia];
-         const int i1 = c_cart.minus[a][ia][d];
-         const int e2 = c_cart.comp[a][ia][d] - 1;
-         double val = pq[d] * V1[m * na1 + i1] + pq[3 + d] * V1[(m + 1) * na1 + i1];
-         if (e2 > 0) {
-            const int i2 = c_cart.minus[a - 1][i1][d];
-            val += e2 * (i2ab * V2[m * na2 + i2] + mrab * V2[(m + 1) * na2 + i2]);
-         }
-         V[idx] = val;
+   const unsigned long long w0 = ((unsigned long long)(set_ij & 0xffff) << 32) | (set_ij >> 16);
+   const unsigned long long w1 = ((unsigned long long)(set_kl & 0xffff) << 32) | (set_kl >> 16);
+   const unsigned long long w2 = ((unsigned long long)n1 << 24) | ((unsigned long long)n2 << 16);
+   const unsigned long long pk = ((ip & 15) << 12) | (((ip >> 4) & 15) << 8) | (((ip >> 8) & 15) << 4) | ((ip >> 12) & 15);
+   const unsigned long long w3 = (pk << 8) | (unsigned long long)n3;
+   unsigned long long z = mix64(w0 + 0x9E3779B97F4A7C15ULL);
+   z = mix64(z ^ w1);
+   z = mix64(z ^ w2);
+   z = mix64(z ^ w3);
+   return z;
+}
+
+// contraction coefficient product of combination l of class k
+__device__ __forceinline__ double kprod(const TypePlan &tp, const GroupRec &g, const double *data, int k, int l,
+                                        uint32_t ip)
+{
+   const int sh = tp.cls_sh[k];
+   const int sa = sh & 15, sb = (sh >> 4) & 15, sc = (sh >> 8) & 15, sd = (sh >> 12) & 15;
+   const int nlb = tp.nl[1][sb], nlc = tp.nl[2][sc], nld = tp.nl[3][sd];
+   const int ia = l / (nlb * nlc * nld), ib = (l / (nlc * nld)) % nlb, ic = (l / nld) % nlc, id = l % nld;
+   const int npa = (g.np4 & 15) + 1, npb = ((g.np4 >> 4) & 15) + 1, npc = ((g.np4 >> 8) & 15) + 1,
+             npd = ((g.np4 >> 12) & 15) + 1;
+   return data[g.Kof[0][sa] + ia * npa + (ip & 15)] * data[g.Kof[1][sb] + ib * npb + ((ip >> 4) & 15)] *
+          data[g.Kof[2][sc] + ic * npc + ((ip >> 8) & 15)] * data[g.Kof[3][sd] + id * npd + ((ip >> 12) & 15)];
+}
+
+// offset in the type's block set of element j of a class's block set
+// (blocks e in [la,LA], f in [lc,LC], e outer, layout [ie][jf] in a block)
+__device__ __forceinline__ int class_src(const TypePlan &tp, int la, int LA, int lc, int LC, int j)
+{
+   for (int e = la; e <= LA; e++) {
+      const int ne = d_nco(e);
+      for (int f = lc; f <= LC; f++) {
+         const int sz = ne * d_nco(f);
+         if (j < sz) return tp.cpoff[e][f] + j;
+         j -= sz;
       }
-      __syncthreads();
    }
-   for (int c = 1; c <= cp.LC; c++) {
-      const int nc = d_nco(c), nc1 = d_nco(c - 1), nc2 = c >= 2 ? d_nco(c - 2) : 1;
-      int total = 0;
-      for (int a = cp.amin[c]; a <= cp.LA; a++) total += (L - a - c + 1) * d_nco(a) * nc;
-      for (int idx0 = lane; idx0 < total; idx0 += nlane) {
-         int idx = idx0, a = cp.amin[c];
-         for (;; a++) {
-            const int sz = (L - a - c + 1) * d_nco(a) * nc;
-            if (idx < sz) break;
-            idx -= sz;
-         }
-         const int na = d_nco(a);
-         const int m = idx / (na * nc);
-         const int rem = idx - m * na * nc;
-         const int ia = rem / nc, ic = rem - ia * nc;
-         const int d = c_cart.first[c][ic];
-         const int j1 = c_cart.minus[c][ic][d];
-         const int f2 = c_cart.comp[c][ic][d] - 1;
-         const double *Vc1 = ws + cp.voff[a][c - 1];
-         double val = pq[6 + d] * Vc1[(m * na + ia) * nc1 + j1] + pq[9 + d] * Vc1[((m + 1) * na + ia) * nc1 + j1];
-         if (f2 > 0) {
-            const int j2 = c_cart.minus[c - 1][j1][d];
-            const double *Vc2 = ws + cp.voff[a][c - 2];
-            val += f2 * (i2cd * Vc2[(m * na + ia) * nc2 + j2] + mrcd * Vc2[((m + 1) * na + ia) * nc2 + j2]);
-         }
-         const int e2 = c_cart.comp[a][ia][d];
-         if (e2 > 0) {
-            const int i1 = c_cart.minus[a][ia][d];
-            const int na1 = d_nco(a - 1);
-            const double *Va = ws + cp.voff[a - 1][c - 1];
-            val += e2 * i2z * Va[((m + 1) * na1 + i1) * nc1 + j1];
-         }
-         ws[cp.voff[a][c] + (m * na + ia) * nc + ic] = val;
-      }
-      __syncthreads();
-   }
+   return 0;
 }
 
 // ---------------------------------------------------------------------------
-// K1: Fm -> VRR -> sum over cells, one 64-lane workgroup per primitive
-// LDS: [ws vws (if in LDS)] [fm 64 x (L+1)] [acc ncp]
+// K1: G_m -> Hermite R -> cell sum -> (e0|f0), one wavefront per primitive
+// quartet.  FIX: one wavefront per listed (primitive, cell) slot, no
+// screening, blocks and per-L Fm[0] into the fix-up workspace.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_fm_vrr(ClassPlan cp, DevTables t, BatchDev b, double *gws)
+struct FmLayout {
+   int SB;      // survivors per Hermite batch
+   int o_sv;    // survivor data, 5 doubles each: X Y Z of P-Q-g, TG1 (or T), TG2
+   int o_R;     // Hermite layers, 2 x SB x nh
+   int o_int;   // byte offset of 64 survivor infos (branch | patch << 2)
+   int bytes;
+};
+
+static FmLayout fm_layout(const TypePlan &tp, bool fix)
+{
+   FmLayout y;
+   const int L = tp.L, nh = tp.nh;
+   int SB = fix ? 1 : (3072 - 64 * (L + 1) - 320) / (2 * nh);
+   SB = std::max(1, std::min(16, SB));
+   y.SB = SB;
+   y.o_sv = 64 * (L + 1);
+   y.o_R = y.o_sv + 320;
+   const int cell_phase = y.o_R + 2 * SB * nh;
+   const int e_phase = nh + 6 * 28 + tp.nbh * tp.nkc;
+   const int dbl = std::max(cell_phase, e_phase);
+   y.o_int = 8 * dbl;
+   y.bytes = y.o_int + 4 * 64;
+   return y;
+}
+
+template <bool FIX>
+__global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, DevTables t, BatchDev b, FmLayout lay)
 {
    extern __shared__ double lds[];
+   const TypePlan &tp = *tpp;
    const int lane = threadIdx.x;
-   const int L = cp.L;
-   double *ws = gws ? gws + (size_t)blockIdx.x * cp.vws : lds;
-   double *fmb = gws ? lds : lds + cp.vws;
-   // cell sums: acc0 takes cell 0, acc the others; out = acc0 + acc, the
-   // order of reference src/compute_ECO.cpp:263-269
-   double *acc = fmb + 64 * (L + 1);
-   double *acc0 = acc + cp.ncp;
-   const bool ssss_src = (cp.la == 0 && cp.lc == 0);
-   unsigned long long n_surv = 0, csum = 0;
-   const unsigned long long wcls = (unsigned long long)((cp.la << 12) | (cp.lb << 8) | (cp.lc << 4) | cp.ld);
+   const int L = tp.L, nh = tp.nh, Lp1 = L + 1;
+   const int SB = lay.SB;
+   double *fmv = lds;                 // [64][L+1]: G_m, then D_m = (-2 rho)^m G_m
+   double *sv = lds + lay.o_sv;       // [64][5]
+   double *Rb = lds + lay.o_R;        // [2][SB][nh]
+   int *svi = (int *)((char *)lds + lay.o_int);
+   const uint32_t nwork = FIX ? min(b.fix_count[0], b.fix_cap) : (b.p1 - b.p0);
 
-   for (uint32_t p = blockIdx.x; p < b.n_prm; p += gridDim.x) {
+   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+      uint32_t p;
+      int fix_n3 = 0;
+      if (FIX) {
+         p = b.fix[2 * w];
+         fix_n3 = (int)(b.fix[2 * w + 1] & 0xff);
+      } else {
+         p = b.p0 + w;
+      }
       const uint32_t r = b.prm_row[p];
       const RowRec row = b.rows[r];
+      const GroupRec &g = b.groups[row.group];
       const uint32_t ip = b.prm_ip[p];
-      const int ipa = ip & 15, ipb = (ip >> 4) & 15, ipc = (ip >> 8) & 15, ipd = (ip >> 12) & 15;
-      const double *data = b.data;
       const int n1 = row.n12 & 0xffff, n2 = row.n12 >> 16;
-      PrimGeo g;
-      prim_prepare(data + row.R[0], data + row.R[1], data + row.R[2], data + row.R[3],
-                   data[row.Z[0] + ipa], data[row.Z[1] + ipb], data[row.Z[2] + ipc], data[row.Z[3] + ipd],
-                   n1, n2, t, g);
-      for (int j = lane; j < cp.ncp; j += 64) acc[j] = acc0[j] = 0.0;
+      const double *data = b.data;
+      PrimGeo geo;
+      prim_prepare(data + g.R[0], data + g.R[1], data + g.R[2], data + g.R[3], data[g.Z[0] + (ip & 15)],
+                   data[g.Z[1] + ((ip >> 4) & 15)], data[g.Z[2] + ((ip >> 8) & 15)], data[g.Z[3] + ((ip >> 12) & 15)],
+                   n1, n2, t, geo);
+      const double m2rho = -2.0 * geo.rho;
+      double acc[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) acc[i] = 0.0;
+      double quirk = 0.0;
+      unsigned long long n_surv = 0, csum = 0, n_any = 0;
+      const unsigned all = (1u << tp.nL) - 1u;
+      const int ncell = FIX ? 1 : t.Ng;
 
-      for (int base = 0; base < t.Ng; base += 64) {
-         const int n3 = base + lane;
-         bool surv = false;
-         if (n3 < t.Ng) {
-            double Fm[MAXLL + 1];
-            slot_fm(t, L, g, n3, Fm);
-            for (int k = 0; k <= L; k++) fmb[lane * (L + 1) + k] = Fm[k];
-            surv = Fm[0] > 1.e-12;
-         }
-         const unsigned long long mask = __ballot(surv);
-         if (surv && b.stats) {
-            // key of the slot, same mixing as the oracle's slot key
-            const unsigned long long w0 = ((unsigned long long)(row.set_ij & 0xffff) << 32) | (row.set_ij >> 16);
-            const unsigned long long w1 = ((unsigned long long)(row.set_kl & 0xffff) << 32) | (row.set_kl >> 16);
-            const unsigned long long w2 = ((unsigned long long)n1 << 24) | ((unsigned long long)n2 << 16) | wcls;
-            const unsigned long long w3 = ((unsigned long long)((ipa << 12) | (ipb << 8) | (ipc << 4) | ipd) << 8) |
-                                          (unsigned long long)n3;
-            unsigned long long z = mix64(w0 + 0x9E3779B97F4A7C15ULL);
-            z = mix64(z ^ w1);
-            z = mix64(z ^ w2);
-            z = mix64(z ^ w3);
-            n_surv++;
-            csum += z;
-         }
-         __syncthreads();
-         const int nslot = min(64, t.Ng - base);
-         for (int s = 0; s < nslot; s++) {
-            const double *Fm = fmb + s * (L + 1);
-            if ((mask >> s) & 1ull) {
-               const int n3s = base + s;
-               double pq[17];
+      for (int base = 0; base < ncell; base += 64) {
+         const int n3 = FIX ? fix_n3 : base + lane;
+         const bool valid = FIX ? (lane == 0) : (n3 < t.Ng);
+         int status = 0;  // 1 kept by every class, 2 screened by every class, 3 class-dependent
+         unsigned mask = 0;
+         double X[3] = {0.0, 0.0, 0.0}, T = 0.0, Fm0 = 0.0, tg1 = 0.0, tg2 = 0.0;
+         int br = GINT_G_ZERO, patch = 0;
+         double fb[MAXLL + 1];
+         if (valid) {
+            {
+               GINT_NO_CONTRACT
+               X[0] = geo.P[0] - geo.Q[0] - t.neighs[n3 * 3 + 0];
+               X[1] = geo.P[1] - geo.Q[1] - t.neighs[n3 * 3 + 1];
+               X[2] = geo.P[2] - geo.Q[2] - t.neighs[n3 * 3 + 2];
+               const double rpq2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
+               T = geo.rho * rpq2;
+            }
+            br = t.truncated ? gint_tcg_locate(geo.Rr, T, t.grid_lo, t.grid_hi, t.bw, &patch, &tg1, &tg2) : GINT_G_BOYS;
+            if (br == GINT_G_PATCH) {
+               double T1[14], T2[14];
+               gint_cheb14(tg1, T1);
+               gint_cheb14(tg2, T2);
                {
-                  const double gx = t.neighs[n3s * 3 + 0], gy = t.neighs[n3s * 3 + 1], gz = t.neighs[n3s * 3 + 2];
-                  const double Qp[3] = {g.Q[0] + gx, g.Q[1] + gy, g.Q[2] + gz};
-                  double W[3];
-                  for (int k = 0; k < 3; k++) W[k] = (g.zab * g.P[k] + g.zcd * Qp[k]) * g.inv_z;
-                  for (int k = 0; k < 3; k++) {
-                     pq[k] = g.PA[k];
-                     pq[3 + k] = W[k] - g.P[k];
-                     pq[6 + k] = g.QC[k];
-                     pq[9 + k] = W[k] - Qp[k];
-                  }
-                  pq[12] = g.i2ab; pq[13] = g.mrab; pq[14] = g.i2cd; pq[15] = g.mrcd; pq[16] = g.i2z;
+                  GINT_NO_CONTRACT
+                  Fm0 = gint_cheb_sum(t.C0 + (size_t)t.ld_C0 * patch, T1, T2) * geo.Kfac;
                }
-               vrr_slot(cp, ws, Fm, pq, lane, 64);
-               double *a = (n3s == 0) ? acc0 : acc;
-               for (int e = cp.la; e <= cp.LA; e++)
-                  for (int f = cp.lc; f <= cp.LC; f++) {
-                     const int sz = d_nco(e) * d_nco(f);
-                     const double *src = ws + cp.voff[e][f];
-                     double *dst = a + cp.cpoff[e][f];
-                     for (int k = lane; k < sz; k += 64) dst[k] += src[k];
+            } else if (br == GINT_G_BOYS) {
+               gint_boys(L, T, fb, t.ftable, geo.Kfac);
+               Fm0 = fb[0];
+            }
+            if (br == GINT_G_ZERO) {
+               status = 2;
+            } else {
+               mask = Fm0 > 1.e-12 ? all : 0u;
+               // F_0 from the downward recursion depends on the starting
+               // order by rounding only; decide each class L exactly when
+               // it is that close to the threshold
+               if (br == GINT_G_BOYS && tp.nL > 1 && T >= 1.e-13 && T < 12. && fabs(Fm0 - 1.e-12) <= 1.e-20) {
+                  mask = 0u;
+                  for (int i = 0; i < tp.nL; i++) {
+                     double f2[MAXLL + 1];
+                     gint_boys(tp.Ls[i], T, f2, t.ftable, geo.Kfac);
+                     if (f2[0] > 1.e-12) mask |= 1u << i;
                   }
-               __syncthreads();
-            } else if (ssss_src && lane == 0 && base + s < t.Ng) {
-               // a screened slot still carries its ssss(0) value
-               // (reference src/compute_VRR.cpp:384-385)
-               if (base + s == 0) acc0[0] += Fm[0];
-               else acc[0] += Fm[0];
+               }
+               status = mask == all ? 1 : (mask == 0u ? 2 : 3);
+            }
+            if (FIX) {
+               // per-L Fm[0] of the slot for the classes that screen it
+               double *f0 = b.fix_ws + (size_t)w * (tp.ncp + 16) + tp.ncp;
+               for (int i = 0; i < tp.nL; i++) {
+                  double v = Fm0;
+                  if (br == GINT_G_BOYS && T >= 1.e-13 && T < 12.) {
+                     double f2[MAXLL + 1];
+                     gint_boys(tp.Ls[i], T, f2, t.ftable, geo.Kfac);
+                     v = f2[0];
+                  }
+                  f0[i] = v;
+               }
+               status = br == GINT_G_ZERO ? 2 : 1;
+            } else if (b.force_fix && status == 1) {
+               status = 3;
+            }
+         }
+         if (!FIX) {
+            if (status == 2) {
+               if (tp.has00) quirk += Fm0;
+            } else if (status == 1 || status == 3) {
+               if (b.stats) {
+                  const unsigned long long h = slot_hash(g.set_ij, g.set_kl, n1, n2, ip, n3);
+                  if (mask == all) {
+                     n_surv += (unsigned long long)tp.nclass;
+                     csum += h * tp.mult_all;
+                  } else {
+                     unsigned long long cm = 0;
+                     for (int k = 0; k < tp.nclass; k++)
+                        if ((mask >> tp.cls_gate[k]) & 1u) {
+                           n_surv++;
+                           cm += tp.cls_mult[k];
+                        }
+                     csum += h * cm;
+                  }
+               }
+               if (status == 3) {
+                  const uint32_t idx = atomicAdd(&b.fix_count[0], 1u);
+                  if (idx < b.fix_cap) {
+                     b.fix[2 * idx] = p;
+                     b.fix[2 * idx + 1] = (uint32_t)n3 | (mask << 8);
+                  } else {
+                     atomicOr(&b.fix_count[1], 1u);
+                  }
+               } else {
+                  n_any++;
+               }
+            }
+         }
+         // compact the survivors of this pass
+         const unsigned long long smask = __ballot(status == 1);
+         const int nsv = __popcll(smask);
+         if (status == 1) {
+            const int s = __popcll(smask & ((1ull << lane) - 1ull));
+            sv[s * 5 + 0] = X[0];
+            sv[s * 5 + 1] = X[1];
+            sv[s * 5 + 2] = X[2];
+            sv[s * 5 + 3] = br == GINT_G_PATCH ? tg1 : T;
+            sv[s * 5 + 4] = tg2;
+            svi[s] = br | (patch << 2);
+            if (br == GINT_G_BOYS) {
+               for (int m = 0; m <= L; m++) fmv[s * Lp1 + m] = fb[m];
+            } else {
+               fmv[s * Lp1] = Fm0;
             }
          }
          __syncthreads();
+         if (nsv == 0) continue;
+         // G_1..G_L of the patch survivors, one (survivor, order) per lane
+         for (int idx = lane; idx < nsv * L; idx += 64) {
+            const int s = idx / L, m = idx - s * L + 1;
+            const int info = svi[s];
+            if ((info & 3) == GINT_G_PATCH) {
+               double T1[14], T2[14];
+               gint_cheb14(sv[s * 5 + 3], T1);
+               gint_cheb14(sv[s * 5 + 4], T2);
+               const double *c = t.C0 + (size_t)t.ld_C0 * (info >> 2) + m * TCG_NCOEF;
+               fmv[s * Lp1 + m] = gint_cheb_sum(c, T1, T2) * geo.Kfac;
+            }
+         }
+         __syncthreads();
+         for (int idx = lane; idx < nsv * Lp1; idx += 64) {
+            const int m = idx % Lp1;
+            double pw = 1.0;
+            for (int k = 0; k < m; k++) pw *= m2rho;
+            fmv[idx] *= pw;
+         }
+         __syncthreads();
+         // Hermite recursion R^(n)_tuv = coef R^(n+1)_{tuv-2d} + X_d R^(n+1)_{tuv-d}
+         for (int s0 = 0; s0 < nsv; s0 += SB) {
+            const int nb = min(SB, nsv - s0);
+            double *prv = Rb, *cur = Rb + SB * nh;
+            if (lane < nb) prv[lane * nh] = fmv[(s0 + lane) * Lp1 + L];
+            __syncthreads();
+            for (int n = L - 1; n >= 1; n--) {
+               const int mn = d_nherm(L - n);
+               for (int h = lane; h < mn; h += 64) {
+                  if (h == 0) {
+                     for (int s = 0; s < nb; s++) cur[s * nh] = fmv[(s0 + s) * Lp1 + n];
+                  } else {
+                     const HermEntry e = c_herm.e[h];
+                     const double cf = (double)e.coef;
+                     for (int s = 0; s < nb; s++) {
+                        double v = sv[(s0 + s) * 5 + e.dir] * prv[s * nh + e.m1];
+                        if (e.m2 >= 0) v += cf * prv[s * nh + e.m2];
+                        cur[s * nh + h] = v;
+                     }
+                  }
+               }
+               __syncthreads();
+               double *tmp = prv;
+               prv = cur;
+               cur = tmp;
+            }
+            // the n = 0 layer goes straight into the cell sum
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+               const int h = lane + 64 * i;
+               if (h < nh) {
+                  double a = acc[i];
+                  if (h == 0) {
+                     for (int s = 0; s < nb; s++) a += fmv[(s0 + s) * Lp1];
+                  } else {
+                     const HermEntry e = c_herm.e[h];
+                     const double cf = (double)e.coef;
+                     for (int s = 0; s < nb; s++) {
+                        double v = sv[(s0 + s) * 5 + e.dir] * prv[s * nh + e.m1];
+                        if (e.m2 >= 0) v += cf * prv[s * nh + e.m2];
+                        a += v;
+                     }
+                  }
+                  acc[i] = a;
+               }
+            }
+            __syncthreads();
+         }
       }
-      double *out = b.ac0 + (size_t)p * cp.ncp;
-      for (int j = lane; j < cp.ncp; j += 64) out[j] = acc0[j] + acc[j];
+
+      // ---- Hermite -> cartesian, once per primitive quartet ----
+      double *sR = lds;                  // [nh]
+      double *Ex = sR + nh;              // [3][28] bra E^{e0}_t
+      double *Fk = Ex + 84;              // [3][28] ket (-1)^tau E^{f0}_tau
+      double *W = Fk + 84;               // [nbh][nkc]
       __syncthreads();
-   }
-   if (b.stats) {
-      for (int off = 32; off > 0; off >>= 1) {
-         n_surv += __shfl_down(n_surv, off, 64);
-         csum += __shfl_down(csum, off, 64);
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+         const int h = lane + 64 * i;
+         if (h < nh) sR[h] = acc[i];
       }
-      if (lane == 0 && n_surv) {
-         atomicAdd(&b.stats[0], n_surv);
-         atomicAdd(&b.stats[1], csum);
-         atomicAdd(&b.stats[2], n_surv * (unsigned long long)(cp.fm_flops + cp.vrr_flops));
+      for (int off = 32; off > 0; off >>= 1) quirk += __shfl_down(quirk, off, 64);
+      quirk = __shfl(quirk, 0, 64);
+      if (lane < 6) {
+         const int d = lane % 3;
+         const bool ket = lane >= 3;
+         const int Lm = ket ? tp.LC : tp.LA;
+         const double x = ket ? geo.QC[d] : geo.PA[d];
+         const double i2 = ket ? geo.i2cd : geo.i2ab;
+         double *E = (ket ? Fk : Ex) + d * 28;
+         E[0] = 1.0;
+         for (int e = 0; e < Lm; e++) {
+            const double *Ee = E + e * (e + 1) / 2;
+            double *En = E + (e + 1) * (e + 2) / 2;
+            for (int tt = 0; tt <= e + 1; tt++) {
+               double v = 0.0;
+               if (tt > 0) v += i2 * Ee[tt - 1];
+               if (tt <= e) v += x * Ee[tt];
+               if (tt + 1 <= e) v += (double)(tt + 1) * Ee[tt + 1];
+               En[tt] = v;
+            }
+         }
+         if (ket)
+            for (int e = 0; e <= Lm; e++)
+               for (int tt = 1; tt <= e; tt += 2) E[e * (e + 1) / 2 + tt] = -E[e * (e + 1) / 2 + tt];
+      }
+      __syncthreads();
+      const int nkc = tp.nkc;
+      for (int idx = lane; idx < tp.nbh * nkc; idx += 64) {
+         const int hb = idx / nkc, jf = idx - hb * nkc;
+         const int tt = c_herm.tuv[hb][0], uu = c_herm.tuv[hb][1], vv = c_herm.tuv[hb][2];
+         int f = tp.LC0;
+         while (jf >= tp.kcoff[f] + d_nco(f)) f++;
+         const int ic = jf - tp.kcoff[f];
+         const int fx = c_cart.comp[f][ic][0], fy = c_cart.comp[f][ic][1], fz = c_cart.comp[f][ic][2];
+         const double *Fx = Fk + fx * (fx + 1) / 2, *Fy = Fk + 28 + fy * (fy + 1) / 2, *Fz = Fk + 56 + fz * (fz + 1) / 2;
+         double s = 0.0;
+         for (int a = 0; a <= fx; a++)
+            for (int c = 0; c <= fy; c++) {
+               const double cxy = Fx[a] * Fy[c];
+               for (int d = 0; d <= fz; d++) s += cxy * Fz[d] * sR[c_herm.idx[tt + a][uu + c][vv + d]];
+            }
+         W[idx] = s;
+      }
+      __syncthreads();
+      double *dst = FIX ? b.fix_ws + (size_t)w * (tp.ncp + 16) : b.ac0 + (size_t)(p - b.p0) * tp.ncp;
+      for (int j = lane; j < tp.ncp; j += 64) {
+         int rem = j, e = tp.LA0;
+         while (rem >= d_nco(e) * nkc) {
+            rem -= d_nco(e) * nkc;
+            e++;
+         }
+         int f = tp.LC0;
+         while (rem >= d_nco(e) * d_nco(f)) {
+            rem -= d_nco(e) * d_nco(f);
+            f++;
+         }
+         const int ie = rem / d_nco(f), jl = rem - ie * d_nco(f);
+         const int ex = c_cart.comp[e][ie][0], ey = c_cart.comp[e][ie][1], ez = c_cart.comp[e][ie][2];
+         const double *Ea = Ex + ex * (ex + 1) / 2, *Eb = Ex + 28 + ey * (ey + 1) / 2, *Ec = Ex + 56 + ez * (ez + 1) / 2;
+         const int col = tp.kcoff[f] + jl;
+         double s = 0.0;
+         for (int a = 0; a <= ex; a++)
+            for (int c = 0; c <= ey; c++) {
+               const double cxy = Ea[a] * Eb[c];
+               for (int d = 0; d <= ez; d++) s += cxy * Ec[d] * W[c_herm.idx[a][c][d] * nkc + col];
+            }
+         if (!FIX && j == 0 && tp.has00) s += quirk;
+         dst[j] = s;
+      }
+      __syncthreads();
+
+      if (!FIX && b.stats) {
+         for (int off = 32; off > 0; off >>= 1) {
+            n_surv += __shfl_down(n_surv, off, 64);
+            csum += __shfl_down(csum, off, 64);
+            n_any += __shfl_down(n_any, off, 64);
+         }
+         if (lane == 0) {
+            if (n_surv) {
+               atomicAdd(&b.stats[0], n_surv);
+               atomicAdd(&b.stats[1], csum);
+            }
+            atomicAdd(&b.stats[2], n_any * (unsigned long long)tp.surv_flops + (unsigned long long)tp.prim_flops);
+         }
       }
    }
 }
 
 // ---------------------------------------------------------------------------
-// K2: contraction over primitives, one 256-thread workgroup per row.
-// out[(e_off + l) * ncp + j] = sum_p Ka Kb Kc Kd (p) * ac0[p][j]
+// fix-up: add a listed slot's blocks to the classes that keep it and its
+// Fm[0] (the quirk of reference src/compute_VRR.cpp:384-385) to the
+// classes with la = lc = 0 that screen it
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_fix_eco(const TypePlan *__restrict__ tpp, BatchDev b)
+{
+   const TypePlan &tp = *tpp;
+   const uint32_t n = min(b.fix_count[0], b.fix_cap);
+   for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+      const uint32_t p = b.fix[2 * w];
+      const unsigned mask = b.fix[2 * w + 1] >> 8;
+      const uint32_t r = b.prm_row[p];
+      const RowRec row = b.rows[r];
+      const GroupRec &g = b.groups[row.group];
+      const uint32_t ip = b.prm_ip[p];
+      const double *src = b.fix_ws + (size_t)w * (tp.ncp + 16);
+      double *erow = b.eco + (size_t)(r - b.r0) * tp.ebytes;
+      for (int k = 0; k < tp.nclass; k++) {
+         const int sh = tp.cls_sh[k];
+         const int la = tp.l[0][sh & 15], lb = tp.l[1][(sh >> 4) & 15], lc = tp.l[2][(sh >> 8) & 15],
+                   ld = tp.l[3][(sh >> 12) & 15];
+         const int nq = tp.cls_nl[k], ncpk = tp.cls_ncp[k];
+         double *out = erow + tp.cls_eoff[k];
+         if ((mask >> tp.cls_gate[k]) & 1u) {
+            for (int idx = threadIdx.x; idx < nq * ncpk; idx += blockDim.x) {
+               const int l = idx / ncpk, j = idx - l * ncpk;
+               const double v = src[class_src(tp, la, la + lb, lc, lc + ld, j)];
+               unsafeAtomicAdd(&out[idx], kprod(tp, g, b.data, k, l, ip) * v);
+            }
+         } else if (la == 0 && lc == 0) {
+            for (int l = threadIdx.x; l < nq; l += blockDim.x)
+               unsafeAtomicAdd(&out[l * ncpk], kprod(tp, g, b.data, k, l, ip) * src[tp.ncp + tp.cls_gate[k]]);
+         }
+      }
+   }
+}
+
+// ---------------------------------------------------------------------------
+// K2: contraction over primitives, one 256-thread workgroup per row, all
+// classes of the type.  eco[row][eoff_k + l ncp_k + j] =
+//    sum_p Ka Kb Kc Kd (p, l) ac0[p][class_src(j)]
 // ---------------------------------------------------------------------------
 constexpr int ECO_THREADS = 256;
 constexpr int ECO_PCHUNK = 16;
 constexpr int ECO_PER_THREAD = 4;
 
-__global__ __launch_bounds__(ECO_THREADS) void k_eco(ClassPlan cp, BatchDev b, uint32_t row_begin, uint32_t row_end,
-                                                      uint32_t prm_begin, uint32_t prm_end)
+__global__ __launch_bounds__(ECO_THREADS) void k_eco(const TypePlan *__restrict__ tpp, BatchDev b)
 {
    __shared__ double sK[MAXNL * MAXNL * MAXNL * MAXNL * ECO_PCHUNK];
+   const TypePlan &tp = *tpp;
    const int tid = threadIdx.x;
-   const int ncp = cp.ncp;
-   for (uint32_t r = row_begin + blockIdx.x; r < row_end; r += gridDim.x) {
-      const RowRec row = b.rows[r];
-      const uint32_t pa = row.prm_off > prm_begin ? row.prm_off : prm_begin;
-      const uint32_t pb = row.prm_off + row.n_prm < prm_end ? row.prm_off + row.n_prm : prm_end;
-      if (pa >= pb) continue;
-      const int nla = row.nlnp & 15, nlb = (row.nlnp >> 4) & 15, nlc = (row.nlnp >> 8) & 15, nld = (row.nlnp >> 12) & 15;
-      const int npa = ((row.nlnp >> 16) & 15) + 1, npb = ((row.nlnp >> 20) & 15) + 1;
-      const int npc = ((row.nlnp >> 24) & 15) + 1, npd = ((row.nlnp >> 28) & 15) + 1;
-      const int nlabcd = nla * nlb * nlc * nld;
-      const double *Ka = b.data + row.K[0], *Kb = b.data + row.K[1], *Kc = b.data + row.K[2], *Kd = b.data + row.K[3];
-      const int total = nlabcd * ncp;
-      double *out = b.eco + (size_t)row.e_off * ncp;
+   const uint32_t r = b.r0 + blockIdx.x;
+   const RowRec row = b.rows[r];
+   const GroupRec &g = b.groups[row.group];
+   const int ncp = tp.ncp;
+   double *erow = b.eco + (size_t)(r - b.r0) * tp.ebytes;
+   const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * ncp;
+   const uint32_t *IP = b.prm_ip + row.prm_off;
+   const int nprm = (int)row.n_prm;
+   for (int k = 0; k < tp.nclass; k++) {
+      const int sh = tp.cls_sh[k];
+      const int la = tp.l[0][sh & 15], lb = tp.l[1][(sh >> 4) & 15], lc = tp.l[2][(sh >> 8) & 15],
+                ld = tp.l[3][(sh >> 12) & 15];
+      const int nq = tp.cls_nl[k], ncpk = tp.cls_ncp[k];
+      const int total = nq * ncpk;
       for (int tile = 0; tile < total; tile += ECO_THREADS * ECO_PER_THREAD) {
-         double accv[ECO_PER_THREAD] = {0.0, 0.0, 0.0, 0.0};
-         for (uint32_t p0 = pa; p0 < pb; p0 += ECO_PCHUNK) {
-            const int np = (int)min((uint32_t)ECO_PCHUNK, pb - p0);
+         double accv[ECO_PER_THREAD];
+         int src[ECO_PER_THREAD], lq[ECO_PER_THREAD];
+#pragma unroll
+         for (int u = 0; u < ECO_PER_THREAD; u++) {
+            accv[u] = 0.0;
+            const int idx = tile + u * ECO_THREADS + tid;
+            lq[u] = idx / ncpk;
+            src[u] = idx < total ? class_src(tp, la, la + lb, lc, lc + ld, idx - lq[u] * ncpk) : 0;
+         }
+         for (int p0 = 0; p0 < nprm; p0 += ECO_PCHUNK) {
+            const int np = min(ECO_PCHUNK, nprm - p0);
             __syncthreads();
-            for (int idx = tid; idx < nlabcd * ECO_PCHUNK; idx += ECO_THREADS) {
+            for (int idx = tid; idx < nq * ECO_PCHUNK; idx += ECO_THREADS) {
                const int l = idx / ECO_PCHUNK, pp = idx - l * ECO_PCHUNK;
-               double kv = 0.0;
-               if (pp < np) {
-                  const uint32_t ip = b.prm_ip[p0 + pp];
-                  const int ipa = ip & 15, ipb = (ip >> 4) & 15, ipc = (ip >> 8) & 15, ipd = (ip >> 12) & 15;
-                  const int a = l / (nlb * nlc * nld), bb = (l / (nlc * nld)) % nlb, c = (l / nld) % nlc, d = l % nld;
-                  kv = Ka[a * npa + ipa] * Kb[bb * npb + ipb] * Kc[c * npc + ipc] * Kd[d * npd + ipd];
-               }
-               sK[idx] = kv;
+               sK[idx] = pp < np ? kprod(tp, g, b.data, k, l, IP[p0 + pp]) : 0.0;
             }
             __syncthreads();
-            const double *A = b.ac0 + (size_t)(p0 - prm_begin) * ncp;
+            const double *A = A0 + (size_t)p0 * ncp;
+#pragma unroll
             for (int u = 0; u < ECO_PER_THREAD; u++) {
                const int idx = tile + u * ECO_THREADS + tid;
                if (idx < total) {
-                  const int l = idx / ncp, j = idx - l * ncp;
                   double s = accv[u];
-                  for (int pp = 0; pp < np; pp++) s += sK[l * ECO_PCHUNK + pp] * A[(size_t)pp * ncp + j];
+                  for (int pp = 0; pp < np; pp++) s += sK[lq[u] * ECO_PCHUNK + pp] * A[(size_t)pp * ncp + src[u]];
                   accv[u] = s;
                }
             }
          }
+#pragma unroll
          for (int u = 0; u < ECO_PER_THREAD; u++) {
             const int idx = tile + u * ECO_THREADS + tid;
-            if (idx < total) out[idx] += accv[u];
+            if (idx < total) erow[tp.cls_eoff[k] + idx] = accv[u];
          }
       }
    }
 }
 
 // ---------------------------------------------------------------------------
-// K3: HRR over the rows of a quartet, sum, cart->sph, K update.
-// workspace: three buffers of cap doubles + acc (Nc)
+// K3: per (set quartet, class): HRR over its rows, sum, cart->sph, K update
 // ---------------------------------------------------------------------------
 constexpr int Q_THREADS = 64;
 
-__device__ void row_ab_cd(const RowRec &row, const double *data, const DevTables &t, double AB[3], double CD[3])
+__device__ void row_ab_cd(const GroupRec &g, const RowRec &row, const double *data, const DevTables &t, double AB[3],
+                          double CD[3])
 {
-   const double *Ao = data + row.R[0], *Bo = data + row.R[1], *Co = data + row.R[2], *Do = data + row.R[3];
+   const double *Ao = data + g.R[0], *Bo = data + g.R[1], *Co = data + g.R[2], *Do = data + g.R[3];
    const int n1 = row.n12 & 0xffff, n2 = row.n12 >> 16;
    double ABs[3], CDs[3];
    gint_pbc(Ao, Bo, t.cell, ABs);
@@ -473,121 +746,169 @@ __device__ void row_ab_cd(const RowRec &row, const double *data, const DevTables
    }
 }
 
-__global__ __launch_bounds__(Q_THREADS) void k_hrr_sph_ks(ClassPlan cp, DevTables t, BatchDev b, KTargets kt, double *gws)
+// block offsets of contraction l in an affine group
+__device__ __forceinline__ void ks_offsets(const KsGroup &ks, int nlb, int nlc, int nld, int l, size_t off[4])
+{
+   const int ia = l / (nlb * nlc * nld), ib = (l / (nlc * nld)) % nlb, ic = (l / nld) % nlc, id = l % nld;
+   off[0] = (size_t)((long long)ks.base[0] + (long long)ia * ks.step[0] + (long long)ic * ks.step[1]);
+   off[1] = (size_t)((long long)ks.base[1] + (long long)ia * ks.step[2] + (long long)id * ks.step[3]);
+   off[2] = (size_t)((long long)ks.base[2] + (long long)ib * ks.step[4] + (long long)ic * ks.step[5]);
+   off[3] = (size_t)((long long)ks.base[3] + (long long)ib * ks.step[6] + (long long)id * ks.step[7]);
+}
+
+__device__ __forceinline__ unsigned blk_idx(bool T, unsigned ld, unsigned row_i, unsigned col_i)
+{
+   return T ? row_i * ld + col_i : col_i * ld + row_i;
+}
+
+__global__ __launch_bounds__(Q_THREADS) void k_hrr_sph_ks(ClassPlan cp, int cls_k, const TypePlan *__restrict__ tpp,
+                                                          DevTables t, BatchDev b, KTargets kt, double *gws, int icap)
 {
    extern __shared__ double lds[];
+   const TypePlan &tp = *tpp;
    const int tid = threadIdx.x;
    const int cap = cp.ycap;
    double *ws = gws ? gws + (size_t)blockIdx.x * cp.hrr_ws : lds;
+   double *Iall = gws ? lds : lds + cp.hrr_ws;
    double *buf[3] = {ws, ws + cap, ws + 2 * cap};
    double *acc = ws + 3 * cap;
-   const int la = cp.la, lb = cp.lb, lc = cp.lc, ld = cp.ld, LA = cp.LA, LC = cp.LC;
-   const int Nc = cp.Nc;
+   const int la = cp.la, lb = cp.lb, lc = cp.lc, ld = cp.ld, LC = cp.LC;
+   const int Nc = cp.Nc, Ns = cp.Ns;
    const int ncd = d_nco(lc) * d_nco(ld);
+   const int sh = tp.cls_sh[cls_k];
+   const int nla = tp.nl[0][sh & 15], nlb = tp.nl[1][(sh >> 4) & 15], nlc = tp.nl[2][(sh >> 8) & 15],
+             nld = tp.nl[3][(sh >> 12) & 15];
+   const int nq_all = nla * nlb * nlc * nld;
+   const int eoff = tp.cls_eoff[cls_k];
+   const int nsa = 2 * la + 1, nsb = 2 * lb + 1, nsc = 2 * lc + 1, nsd = 2 * ld + 1;
+   const int nscd = nsc * nsd, nsbcd = nsb * nscd;
+   const double symm[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
 
-   for (uint32_t q = blockIdx.x; q < b.n_q; q += gridDim.x) {
-      const KsRec ks = b.ks[q];
-      for (int i = tid; i < Nc; i += Q_THREADS) acc[i] = 0.0;
-      for (uint32_t r = ks.row0; r < ks.row1; r++) {
-         const RowRec row = b.rows[r];
-         double AB[3], CD[3];
-         row_ab_cd(row, b.data, t, AB, CD);
-         const double *E = b.eco + (size_t)(row.e_off + ks.l) * cp.ncp;
+   for (uint32_t gi = b.g0 + blockIdx.x; gi < b.g1; gi += gridDim.x) {
+      const KsGroup ks = b.ksg[(size_t)gi * tp.nclass + cls_k];
+      const int nq = (int)(ks.flags >> 16);
+      if (nq == 0) continue;
+      const GroupRec &g = b.groups[gi];
+      const bool expl = (ks.flags & KS_EXPLICIT) != 0;
+      const bool reduce = !expl && nq == nq_all && nq * Ns <= icap;
+      for (int l = 0; l < nq; l++) {
          __syncthreads();
-         for (int i = tid; i < cp.ncp; i += Q_THREADS) buf[0][i] = E[i];
-         __syncthreads();
-         int src = 0;
-         // ket HRR (d -> c): layer dd from layer dd-1, blocks (e,c) e outer
-         for (int dd = 1; dd <= ld; dd++) {
-            const int dst = (src + 1) % 3;
-            const int nd = d_nco(dd), ndm = d_nco(dd - 1);
-            for (int idx0 = tid; idx0 < cp.ksize[dd]; idx0 += Q_THREADS) {
-               int e = la, c = lc;
-               // locate the block (blocks are few: at most 4 x 4)
-               for (;;) {
-                  const int end_ec = cp.koff[dd][e][c] + d_nco(e) * d_nco(c) * nd;
-                  if (idx0 < end_ec) break;
-                  if (++c > LC - dd) { c = lc; e++; }
+         for (int i = tid; i < Nc; i += Q_THREADS) acc[i] = 0.0;
+         for (uint32_t r = g.row0; r < g.row1; r++) {
+            const RowRec row = b.rows[r];
+            double AB[3], CD[3];
+            row_ab_cd(g, row, b.data, t, AB, CD);
+            const double *E = b.eco + (size_t)(r - b.r0) * tp.ebytes + eoff + (size_t)l * cp.ncp;
+            __syncthreads();
+            for (int i = tid; i < cp.ncp; i += Q_THREADS) buf[0][i] = E[i];
+            __syncthreads();
+            int src = 0;
+            // ket HRR (d -> c): layer dd from layer dd-1, blocks (e,c) e outer
+            for (int dd = 1; dd <= ld; dd++) {
+               const int dst = (src + 1) % 3;
+               const int nd = d_nco(dd), ndm = d_nco(dd - 1);
+               for (int idx0 = tid; idx0 < cp.ksize[dd]; idx0 += Q_THREADS) {
+                  int e = la, c = lc;
+                  for (;;) {
+                     const int end_ec = cp.koff[dd][e][c] + d_nco(e) * d_nco(c) * nd;
+                     if (idx0 < end_ec) break;
+                     if (++c > LC - dd) {
+                        c = lc;
+                        e++;
+                     }
+                  }
+                  const int idx = idx0 - cp.koff[dd][e][c];
+                  const int ncc = d_nco(c), ncc1 = d_nco(c + 1);
+                  const int ie = idx / (ncc * nd);
+                  const int rem = idx - ie * ncc * nd;
+                  const int ic = rem / nd, id = rem - ic * nd;
+                  const int dir = c_cart.first[dd][id];
+                  const int idm = c_cart.minus[dd][id][dir];
+                  const int icp = c_cart.plus[c][ic][dir];
+                  const double v1 = buf[src][cp.koff[dd - 1][e][c + 1] + (ie * ncc1 + icp) * ndm + idm];
+                  const double v0 = buf[src][cp.koff[dd - 1][e][c] + (ie * ncc + ic) * ndm + idm];
+                  buf[dst][idx0] = v1 + CD[dir] * v0;
                }
-               const int idx = idx0 - cp.koff[dd][e][c];
-               const int ncc = d_nco(c), ncc1 = d_nco(c + 1);
-               const int ie = idx / (ncc * nd);
-               const int rem = idx - ie * ncc * nd;
-               const int ic = rem / nd, id = rem - ic * nd;
-               const int dir = c_cart.first[dd][id];
-               const int idm = c_cart.minus[dd][id][dir];
-               const int icp = c_cart.plus[c][ic][dir];
-               const double v1 = buf[src][cp.koff[dd - 1][e][c + 1] + (ie * ncc1 + icp) * ndm + idm];
-               const double v0 = buf[src][cp.koff[dd - 1][e][c] + (ie * ncc + ic) * ndm + idm];
-               buf[dst][idx0] = v1 + CD[dir] * v0;
+               __syncthreads();
+               src = dst;
             }
-            __syncthreads();
-            src = dst;
-         }
-         // bra HRR (b -> a): layer bb from layer bb-1, blocks a in [la, LA-bb]
-         for (int bb = 1; bb <= lb; bb++) {
-            const int dst = (src + 1) % 3;
-            const int nb = d_nco(bb), nbm = d_nco(bb - 1);
-            for (int idx0 = tid; idx0 < cp.bsize[bb]; idx0 += Q_THREADS) {
-               int a = la;
-               while (idx0 >= cp.boff[bb][a] + d_nco(a) * nb * ncd) a++;
-               const int idx = idx0 - cp.boff[bb][a];
-               const int ia = idx / (nb * ncd);
-               const int rem = idx - ia * nb * ncd;
-               const int ib = rem / ncd, icd = rem - ib * ncd;
-               const int dir = c_cart.first[bb][ib];
-               const int ibm = c_cart.minus[bb][ib][dir];
-               const int iap = c_cart.plus[a][ia][dir];
-               const double v1 = buf[src][cp.boff[bb - 1][a + 1] + (iap * nbm + ibm) * ncd + icd];
-               const double v0 = buf[src][cp.boff[bb - 1][a] + (ia * nbm + ibm) * ncd + icd];
-               buf[dst][idx0] = v1 + AB[dir] * v0;
+            // bra HRR (b -> a): layer bb from layer bb-1, blocks a in [la, LA-bb]
+            for (int bb = 1; bb <= lb; bb++) {
+               const int dst = (src + 1) % 3;
+               const int nb = d_nco(bb), nbm = d_nco(bb - 1);
+               for (int idx0 = tid; idx0 < cp.bsize[bb]; idx0 += Q_THREADS) {
+                  int a = la;
+                  while (idx0 >= cp.boff[bb][a] + d_nco(a) * nb * ncd) a++;
+                  const int idx = idx0 - cp.boff[bb][a];
+                  const int ia = idx / (nb * ncd);
+                  const int rem = idx - ia * nb * ncd;
+                  const int ib = rem / ncd, icd = rem - ib * ncd;
+                  const int dir = c_cart.first[bb][ib];
+                  const int ibm = c_cart.minus[bb][ib][dir];
+                  const int iap = c_cart.plus[a][ia][dir];
+                  const double v1 = buf[src][cp.boff[bb - 1][a + 1] + (iap * nbm + ibm) * ncd + icd];
+                  const double v0 = buf[src][cp.boff[bb - 1][a] + (ia * nbm + ibm) * ncd + icd];
+                  buf[dst][idx0] = v1 + AB[dir] * v0;
+               }
+               __syncthreads();
+               src = dst;
             }
-            __syncthreads();
-            src = dst;
+            // the first block of the last layer is (la lb|lc ld)
+            for (int i = tid; i < Nc; i += Q_THREADS) acc[i] += buf[src][i];
          }
-         // the first block of the last layer is (la lb|lc ld)
-         for (int i = tid; i < Nc; i += Q_THREADS) acc[i] += buf[src][i];
-      }
-      __syncthreads();
-      // cartesian -> spherical: four stages, each moves the leading index last
-      {
-         const int ls[4] = {la, lb, lc, ld};
-         int dims[4] = {d_nco(la), d_nco(lb), d_nco(lc), d_nco(ld)};
-         const double *in = acc;
-         for (int s = 0; s < 4; s++) {
-            double *o = buf[s & 1];
-            const int k = dims[0];
-            const int n = dims[1] * dims[2] * dims[3];
-            const int m = 2 * ls[s] + 1;
-            const double *c2s = c_c2s + c_c2s_off[ls[s]];
-            for (int idx = tid; idx < n * m; idx += Q_THREADS) {
-               const int j = idx / m, i = idx - j * m;
-               double v = 0.0;
-               for (int qq = 0; qq < k; qq++) v += c2s[i * k + qq] * in[qq * n + j];
-               o[idx] = v;
+         __syncthreads();
+         // cartesian -> spherical: four stages, each moves the leading index last
+         {
+            const int ls[4] = {la, lb, lc, ld};
+            int dims[4] = {d_nco(la), d_nco(lb), d_nco(lc), d_nco(ld)};
+            const double *in = acc;
+            for (int s = 0; s < 4; s++) {
+               double *o = buf[s & 1];
+               const int k = dims[0];
+               const int n = dims[1] * dims[2] * dims[3];
+               const int m = 2 * ls[s] + 1;
+               const double *c2s = c_c2s + c_c2s_off[ls[s]];
+               for (int idx = tid; idx < n * m; idx += Q_THREADS) {
+                  const int j = idx / m, i = idx - j * m;
+                  double v = 0.0;
+                  for (int qq = 0; qq < k; qq++) v += c2s[i * k + qq] * in[qq * n + j];
+                  o[idx] = v;
+               }
+               __syncthreads();
+               in = o;
+               const int nd0 = dims[1], nd1 = dims[2], nd2 = dims[3];
+               dims[0] = nd0;
+               dims[1] = nd1;
+               dims[2] = nd2;
+               dims[3] = m;
             }
-            __syncthreads();
-            in = o;
-            const int nd0 = dims[1], nd1 = dims[2], nd2 = dims[3];
-            dims[0] = nd0; dims[1] = nd1; dims[2] = nd2; dims[3] = m;
          }
-      }
-      const double *I = buf[1];
-      // exchange update, reference src/compute_KS.cpp:51-115
-      {
-         const int nsa = 2 * la + 1, nsb = 2 * lb + 1, nsc = 2 * lc + 1, nsd = 2 * ld + 1;
-         const int nscd = nsc * nsd, nsbcd = nsb * nscd, nsabcd = nsa * nsbcd;
-         const double symm[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
-         const double fac = symm[(ks.flags >> 4) & 7] * kt.hf_fac;
-         const unsigned ldac = ks.ld01 & 0xffff, ldad = ks.ld01 >> 16, ldbc = ks.ld23 & 0xffff, ldbd = ks.ld23 >> 16;
-         for (int tt = tid; tt < nsabcd; tt += Q_THREADS) {
+         const double *I = buf[1];
+         if (reduce) {
+            for (int i = tid; i < Ns; i += Q_THREADS) Iall[l * Ns + i] = I[i];
+            continue;
+         }
+         // one quartet: 4 atomic updates per spherical integral, reference
+         // src/compute_KS.cpp:51-115
+         size_t off[4];
+         uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
+         if (expl) {
+            const KsRec x = b.ksx[ks.expl + l];
+            for (int q = 0; q < 4; q++) off[q] = x.off[q];
+            ld01 = x.ld01;
+            ld23 = x.ld23;
+            flags = x.flags;
+         } else {
+            ks_offsets(ks, nlb, nlc, nld, l, off);
+         }
+         const double fac = symm[(flags >> 4) & 7] * kt.hf_fac;
+         const unsigned ldac = ld01 & 0xffff, ldad = ld01 >> 16, ldbc = ld23 & 0xffff, ldbd = ld23 >> 16;
+         for (int tt = tid; tt < Ns; tt += Q_THREADS) {
             const unsigned isa = tt / nsbcd, isb = (tt / nscd) % nsb, isc = (tt / nsd) % nsc, isd = tt % nsd;
-            const unsigned iac = !(ks.flags & 1) ? isc * ldac + isa : isa * ldac + isc;
-            const unsigned iad = !(ks.flags & 2) ? isd * ldad + isa : isa * ldad + isd;
-            const unsigned ibc = !(ks.flags & 4) ? isc * ldbc + isb : isb * ldbc + isc;
-            const unsigned ibd = !(ks.flags & 8) ? isd * ldbd + isb : isb * ldbd + isd;
-            const size_t Iac = (size_t)ks.off[0] + iac, Iad = (size_t)ks.off[1] + iad;
-            const size_t Ibc = (size_t)ks.off[2] + ibc, Ibd = (size_t)ks.off[3] + ibd;
+            const size_t Iac = off[0] + blk_idx(flags & 1, ldac, isa, isc);
+            const size_t Iad = off[1] + blk_idx(flags & 2, ldad, isa, isd);
+            const size_t Ibc = off[2] + blk_idx(flags & 4, ldbc, isb, isc);
+            const size_t Ibd = off[3] + blk_idx(flags & 8, ldbd, isb, isd);
             const double iabcd = -fac * I[tt];
             for (int s = 0; s < kt.nspin; s++) {
                const double *P = kt.P[s];
@@ -599,7 +920,78 @@ __global__ __launch_bounds__(Q_THREADS) void k_hrr_sph_ks(ClassPlan cp, DevTable
             }
          }
       }
+      if (!reduce) continue;
       __syncthreads();
+      // the same updates, summed over the set quartet's contractions first:
+      // K_bd += i P_ac, K_bc += i P_ad, K_ad += i P_bc, K_ac += i P_bd
+      const double fac = symm[(ks.flags >> 4) & 7] * kt.hf_fac;
+      if (fac == 0.0) continue;
+      const unsigned ldac = ks.ld01 & 0xffff, ldad = ks.ld01 >> 16, ldbc = ks.ld23 & 0xffff, ldbd = ks.ld23 >> 16;
+      const bool Tac = ks.flags & 1, Tad = ks.flags & 2, Tbc = ks.flags & 4, Tbd = ks.flags & 8;
+      const int NA = nla * nsa, NB = nlb * nsb, NC = nlc * nsc, ND = nld * nsd;
+      auto Iat = [&](int ia, int isa, int ib, int isb, int ic, int isc, int id, int isd) {
+         return Iall[(((ia * nlb + ib) * nlc + ic) * nld + id) * Ns + ((isa * nsb + isb) * nsc + isc) * nsd + isd];
+      };
+      auto offs = [&](int ia, int ib, int ic, int id, int q) {
+         switch (q) {
+            case 0: return (long long)ks.base[0] + (long long)ia * ks.step[0] + (long long)ic * ks.step[1];
+            case 1: return (long long)ks.base[1] + (long long)ia * ks.step[2] + (long long)id * ks.step[3];
+            case 2: return (long long)ks.base[2] + (long long)ib * ks.step[4] + (long long)ic * ks.step[5];
+            default: return (long long)ks.base[3] + (long long)ib * ks.step[6] + (long long)id * ks.step[7];
+         }
+      };
+      for (int s = 0; s < kt.nspin; s++) {
+         const double *P = kt.P[s];
+         double *K = kt.K[s];
+         for (int x = tid; x < NB * ND; x += Q_THREADS) {  // K_bd
+            const int ib = (x / ND) / nsb, isb = (x / ND) % nsb, id = (x % ND) / nsd, isd = (x % ND) % nsd;
+            double sum = 0.0;
+            for (int ia = 0; ia < nla; ia++)
+               for (int ic = 0; ic < nlc; ic++) {
+                  const size_t o = (size_t)offs(ia, 0, ic, 0, 0);
+                  for (int isa = 0; isa < nsa; isa++)
+                     for (int isc = 0; isc < nsc; isc++)
+                        sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tac, ldac, isa, isc)];
+               }
+            unsafeAtomicAdd(&K[(size_t)offs(0, ib, 0, id, 3) + blk_idx(Tbd, ldbd, isb, isd)], -fac * sum);
+         }
+         for (int x = tid; x < NB * NC; x += Q_THREADS) {  // K_bc
+            const int ib = (x / NC) / nsb, isb = (x / NC) % nsb, ic = (x % NC) / nsc, isc = (x % NC) % nsc;
+            double sum = 0.0;
+            for (int ia = 0; ia < nla; ia++)
+               for (int id = 0; id < nld; id++) {
+                  const size_t o = (size_t)offs(ia, 0, 0, id, 1);
+                  for (int isa = 0; isa < nsa; isa++)
+                     for (int isd = 0; isd < nsd; isd++)
+                        sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tad, ldad, isa, isd)];
+               }
+            unsafeAtomicAdd(&K[(size_t)offs(0, ib, ic, 0, 2) + blk_idx(Tbc, ldbc, isb, isc)], -fac * sum);
+         }
+         for (int x = tid; x < NA * ND; x += Q_THREADS) {  // K_ad
+            const int ia = (x / ND) / nsa, isa = (x / ND) % nsa, id = (x % ND) / nsd, isd = (x % ND) % nsd;
+            double sum = 0.0;
+            for (int ib = 0; ib < nlb; ib++)
+               for (int ic = 0; ic < nlc; ic++) {
+                  const size_t o = (size_t)offs(0, ib, ic, 0, 2);
+                  for (int isb = 0; isb < nsb; isb++)
+                     for (int isc = 0; isc < nsc; isc++)
+                        sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tbc, ldbc, isb, isc)];
+               }
+            unsafeAtomicAdd(&K[(size_t)offs(ia, 0, 0, id, 1) + blk_idx(Tad, ldad, isa, isd)], -fac * sum);
+         }
+         for (int x = tid; x < NA * NC; x += Q_THREADS) {  // K_ac
+            const int ia = (x / NC) / nsa, isa = (x / NC) % nsa, ic = (x % NC) / nsc, isc = (x % NC) % nsc;
+            double sum = 0.0;
+            for (int ib = 0; ib < nlb; ib++)
+               for (int id = 0; id < nld; id++) {
+                  const size_t o = (size_t)offs(0, ib, 0, id, 3);
+                  for (int isb = 0; isb < nsb; isb++)
+                     for (int isd = 0; isd < nsd; isd++)
+                        sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tbd, ldbd, isb, isd)];
+               }
+            unsafeAtomicAdd(&K[(size_t)offs(ia, 0, ic, 0, 0) + blk_idx(Tac, ldac, isa, isc)], -fac * sum);
+         }
+      }
    }
 }
 
@@ -608,38 +1000,68 @@ __global__ __launch_bounds__(Q_THREADS) void k_hrr_sph_ks(ClassPlan cp, DevTable
 // ---------------------------------------------------------------------------
 static constexpr size_t LDS_LIMIT = 64 * 1024;
 
-static size_t vrr_lds_full(const ClassPlan &cp) { return sizeof(double) * ((size_t)cp.vws + 64 * (cp.L + 1) + 2 * cp.ncp); }
-static size_t vrr_lds_part(const ClassPlan &cp) { return sizeof(double) * ((size_t)64 * (cp.L + 1) + 2 * cp.ncp); }
+size_t fm_r_lds_bytes(const TypePlan &tp)
+{
+   const FmLayout y = fm_layout(tp, false);
+   return (size_t)y.bytes <= LDS_LIMIT ? (size_t)y.bytes : 0;
+}
+
+int launch_fm_r(const TypePlan &tp, const DevTables &t, const BatchDev &b, void *stream)
+{
+   if (b.p1 <= b.p0) return 0;
+   const FmLayout y = fm_layout(tp, false);
+   if ((size_t)y.bytes > LDS_LIMIT) return -3;
+   hipLaunchKernelGGL(k_fm_r<false>, dim3(b.p1 - b.p0), dim3(64), y.bytes, (hipStream_t)stream, b.tp, t, b, y);
+   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_fixup(const TypePlan &tp, const DevTables &t, const BatchDev &b, uint32_t n_fix, void *stream)
+{
+   const FmLayout y = fm_layout(tp, true);
+   if ((size_t)y.bytes > LDS_LIMIT) return -3;
+   const unsigned grid = std::max(1u, std::min(n_fix, 4096u));
+   hipLaunchKernelGGL(k_fm_r<true>, dim3(grid), dim3(64), y.bytes, (hipStream_t)stream, b.tp, t, b, y);
+   if (hipGetLastError() != hipSuccess) return -1;
+   hipLaunchKernelGGL(k_fix_eco, dim3(grid), dim3(256), 0, (hipStream_t)stream, b.tp, b);
+   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_eco(const TypePlan &tp, const BatchDev &b, void *stream)
+{
+   (void)tp;
+   if (b.r1 <= b.r0) return 0;
+   hipLaunchKernelGGL(k_eco, dim3(b.r1 - b.r0), dim3(ECO_THREADS), 0, (hipStream_t)stream, b.tp, b);
+   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 static size_t hrr_lds_full(const ClassPlan &cp) { return sizeof(double) * (size_t)cp.hrr_ws; }
 
 size_t scratch_doubles(const ClassPlan &cp)
 {
-   size_t need = 0;
-   if (vrr_lds_full(cp) > LDS_LIMIT) need = std::max(need, (size_t)cp.vws * SCRATCH_GRID);
-   if (hrr_lds_full(cp) > LDS_LIMIT) need = std::max(need, (size_t)cp.hrr_ws * SCRATCH_GRID);
-   return need;
+   return hrr_lds_full(cp) > LDS_LIMIT ? (size_t)cp.hrr_ws * SCRATCH_GRID : 0;
 }
 
-int launch_fm_vrr(const ClassPlan &cp, const DevTables &t, const BatchDev &b, Scratch s, void *stream)
+int launch_hrr_sph_ks(const ClassPlan &cp, int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b,
+                      const KTargets &kt, Scratch s, void *stream)
 {
-   if (b.n_prm == 0) return 0;
+   if (b.g1 <= b.g0) return 0;
    hipStream_t st = (hipStream_t)stream;
-   if (vrr_lds_full(cp) <= LDS_LIMIT) {
-      hipLaunchKernelGGL(k_fm_vrr, dim3(b.n_prm), dim3(64), vrr_lds_full(cp), st, cp, t, b, (double *)nullptr);
+   const uint32_t ng = b.g1 - b.g0;
+   const bool in_lds = hrr_lds_full(cp) <= LDS_LIMIT;
+   const size_t ws_bytes = in_lds ? hrr_lds_full(cp) : 0;
+   const int need = tp.cls_nl[cls_k] * cp.Ns;
+   int icap = (int)((LDS_LIMIT - ws_bytes) / sizeof(double));
+   icap = need <= icap ? need : 0;
+   const size_t lds = ws_bytes + sizeof(double) * (size_t)icap;
+   if (in_lds) {
+      const unsigned grid = std::min(ng, 1u << 20);
+      hipLaunchKernelGGL(k_hrr_sph_ks, dim3(grid), dim3(Q_THREADS), lds, st, cp, cls_k, b.tp, t, b, kt,
+                         (double *)nullptr, icap);
    } else {
-      const unsigned grid = b.n_prm < SCRATCH_GRID ? b.n_prm : SCRATCH_GRID;
-      if (!s.ptr || s.doubles < (size_t)cp.vws * grid) return -2;
-      hipLaunchKernelGGL(k_fm_vrr, dim3(grid), dim3(64), vrr_lds_part(cp), st, cp, t, b, s.ptr);
+      const unsigned grid = std::min(ng, SCRATCH_GRID);
+      if (!s.ptr || s.doubles < (size_t)cp.hrr_ws * grid) return -2;
+      hipLaunchKernelGGL(k_hrr_sph_ks, dim3(grid), dim3(Q_THREADS), lds, st, cp, cls_k, b.tp, t, b, kt, s.ptr, icap);
    }
-   return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_eco_range(const ClassPlan &cp, const BatchDev &b, uint32_t row_begin, uint32_t row_end,
-                     uint32_t prm_begin, uint32_t prm_end, void *stream)
-{
-   if (row_end <= row_begin || prm_end <= prm_begin) return 0;
-   hipLaunchKernelGGL(k_eco, dim3(row_end - row_begin), dim3(ECO_THREADS), 0, (hipStream_t)stream, cp, b, row_begin,
-                      row_end, prm_begin, prm_end);
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -677,22 +1099,6 @@ int device_math_selftest(const double *x, const double *y, int n, double *out)
    if (dy) (void)hipFree(dy);
    if (dout) (void)hipFree(dout);
    return rc;
-}
-
-int launch_hrr_sph_ks(const ClassPlan &cp, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s,
-                      void *stream)
-{
-   if (b.n_q == 0) return 0;
-   hipStream_t st = (hipStream_t)stream;
-   if (hrr_lds_full(cp) <= LDS_LIMIT) {
-      hipLaunchKernelGGL(k_hrr_sph_ks, dim3(b.n_q), dim3(Q_THREADS), hrr_lds_full(cp), st, cp, t, b, kt,
-                         (double *)nullptr);
-   } else {
-      const unsigned grid = b.n_q < SCRATCH_GRID ? b.n_q : SCRATCH_GRID;
-      if (!s.ptr || s.doubles < (size_t)cp.hrr_ws * grid) return -2;
-      hipLaunchKernelGGL(k_hrr_sph_ks, dim3(grid), dim3(Q_THREADS), 0, st, cp, t, b, kt, s.ptr);
-   }
-   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 }  // namespace gint

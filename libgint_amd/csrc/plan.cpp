@@ -1,17 +1,22 @@
-// Per-class recurrence plan: workspace layouts of the vertical (Obara-Saika)
-// and horizontal (Head-Gordon-Pople) recurrences for (la lb|lc ld).
+// Plans of the recurrences.
 //
-// Same recurrence choices as reference src/plan.cpp:56-146 (rec_add): the
-// bra VRR builds a from the (0|0)^(m) ladder, the ket VRR builds c with the
-// cross term into a-1, HRR2 moves b onto a, HRR1 moves d onto c, the
-// cartesian direction is the first non-zero one (x, y, z).  The layout is
-// this library's own: instead of a list of 19-int ops, the kernels walk
-// fixed level loops over blocks whose offsets are computed here.
+// ClassPlan: horizontal (Head-Gordon-Pople) recurrence layers and
+// cartesian -> spherical sizes of one class (la lb|lc ld).  HRR2 moves b
+// onto a, HRR1 moves d onto c, the cartesian direction is the first non-zero
+// one (x, y, z), as in reference src/plan.cpp:56-146 (rec_add).  Instead of
+// a list of 19-int ops the kernels walk fixed level loops over blocks whose
+// offsets are computed here.
+//
+// TypePlan: the union of the (e0|f0) blocks that the classes of one quartet
+// type read, the Hermite sizes of the McMurchie-Davidson path that produces
+// them, and the per-class bookkeeping (gates, contraction layout, checksum
+// multipliers).
 #include <algorithm>
 #include <cstring>
 #include <map>
 #include <set>
 #include <tuple>
+#include <vector>
 
 #include "device.hpp"
 
@@ -84,18 +89,6 @@ ClassPlan make_plan(int la, int lb, int lc, int ld)
       }
    p.ncp = off;
 
-   // VRR blocks: ket level c needs a >= max(0, la - (LC - c)); level 0 needs
-   // the whole bra ladder 0..LA.  Block (a,c) carries m = 0..L-a-c.
-   p.amin[0] = 0;
-   for (int c = 1; c <= p.LC; c++) p.amin[c] = std::max(0, la - (p.LC - c));
-   off = 0;
-   for (int c = 0; c <= p.LC; c++)
-      for (int a = p.amin[c]; a <= p.LA; a++) {
-         p.voff[a][c] = off;
-         off += (p.L - a - c + 1) * nco(a) * nco(c);
-      }
-   p.vws = off;
-
    // HRR layers
    int cap = std::max(p.ncp, p.Nc);
    for (int dd = 0; dd <= ld; dd++) {
@@ -139,6 +132,127 @@ ClassPlan make_plan(int la, int lb, int lc, int ld)
       p.sph_flops = sf;
    }
    return p;
+}
+
+// odd multiplier of a class in the survivor checksum (the oracle uses the
+// same function)
+uint64_t class_mult(int code)
+{
+   uint64_t z = 0xC1A55E5ULL + (uint64_t)code * 0x9E3779B97F4A7C15ULL;
+   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+   z = z ^ (z >> 31);
+   return z | 1ull;
+}
+
+// FLOPs of the Hermite recursion R^(n)_tuv, n = L-1..0, from the G_m ladder:
+// an element whose recursion direction has component >= 2 costs 3 (one
+// product with (t-1), one fused product-add), otherwise 1; plus the (-2 rho)^n
+// scaling (2 per order) and the cell accumulation of the n = 0 layer.
+static long long herm_flops(int L)
+{
+   long long f = 2LL * (L + 1);
+   for (int n = L - 1; n >= 0; n--)
+      for (int N = 1; N <= L - n; N++)
+         for (int t = N; t >= 0; t--)
+            for (int u = N - t; u >= 0; u--) {
+               const int v = N - t - u;
+               const int c = t > 0 ? t : (u > 0 ? u : v);
+               f += c >= 2 ? 3 : 1;
+            }
+   return f + nherm(L);
+}
+
+void make_type_plan(TypePlan &tp)
+{
+   TypePlan p;
+   std::memset(&p, 0, sizeof p);
+   for (int t = 0; t < 4; t++) {
+      p.nsh[t] = tp.nsh[t];
+      for (int s = 0; s < tp.nsh[t]; s++) {
+         p.l[t][s] = tp.l[t][s];
+         p.nl[t][s] = tp.nl[t][s];
+      }
+   }
+   int lmin[4], lmax[4];
+   for (int t = 0; t < 4; t++) {
+      lmin[t] = 99;
+      lmax[t] = -1;
+      for (int s = 0; s < p.nsh[t]; s++) {
+         lmin[t] = std::min(lmin[t], p.l[t][s]);
+         lmax[t] = std::max(lmax[t], p.l[t][s]);
+      }
+   }
+   p.LA0 = lmin[0];
+   p.LA = lmax[0] + lmax[1];
+   p.LC0 = lmin[2];
+   p.LC = lmax[2] + lmax[3];
+   p.L = p.LA + p.LC;
+   int off = 0;
+   for (int e = p.LA0; e <= p.LA; e++)
+      for (int f = p.LC0; f <= p.LC; f++) {
+         p.cpoff[e][f] = off;
+         off += nco(e) * nco(f);
+      }
+   p.ncp = off;
+   off = 0;
+   for (int f = p.LC0; f <= p.LC; f++) {
+      p.kcoff[f] = off;
+      off += nco(f);
+   }
+   p.nkc = off;
+   p.nbh = nherm(p.LA);
+   p.nh = nherm(p.L);
+   p.has00 = (p.LA0 == 0 && p.LC0 == 0) ? 1 : 0;
+
+   std::set<int> Lset;
+   int k = 0, eoff = 0;
+   long long ref = 0;
+   for (int sa = 0; sa < p.nsh[0]; sa++)
+      for (int sb = 0; sb < p.nsh[1]; sb++)
+         for (int sc = 0; sc < p.nsh[2]; sc++)
+            for (int sd = 0; sd < p.nsh[3]; sd++, k++) {
+               const int la = p.l[0][sa], lb = p.l[1][sb], lc = p.l[2][sc], ld = p.l[3][sd];
+               const ClassPlan cp = make_plan(la, lb, lc, ld);
+               p.cls_code[k] = class_code(la, lb, lc, ld);
+               p.cls_sh[k] = sa | sb << 4 | sc << 8 | sd << 12;
+               p.cls_nl[k] = p.nl[0][sa] * p.nl[1][sb] * p.nl[2][sc] * p.nl[3][sd];
+               p.cls_ncp[k] = cp.ncp;
+               p.cls_eoff[k] = eoff;
+               eoff += p.cls_nl[k] * cp.ncp;
+               p.cls_mult[k] = class_mult(p.cls_code[k]);
+               p.mult_all += p.cls_mult[k];
+               Lset.insert(cp.L);
+               ref += cp.fm_flops + cp.vrr_flops;
+            }
+   p.nclass = k;
+   p.ebytes = eoff;
+   p.nL = 0;
+   for (int L : Lset) p.Ls[p.nL++] = L;
+   for (k = 0; k < p.nclass; k++) {
+      const int c = p.cls_code[k];
+      const int L = c / 64 + (c / 16) % 4 + (c / 4) % 4 + c % 4;
+      for (int i = 0; i < p.nL; i++)
+         if (p.Ls[i] == L) p.cls_gate[k] = i;
+   }
+   p.ref_slot_flops = ref;
+   p.surv_flops = 52 + 211LL * (p.L + 1) + herm_flops(p.L);
+   // Hermite -> cartesian: E coefficients (3 directions, 3 FLOPs per entry
+   // on each side), then 3 FLOPs per term of the two contraction stages
+   long long pf = 0;
+   for (int e = 0; e <= p.LA; e++) pf += 3LL * 3 * (e + 1);
+   for (int f = 0; f <= p.LC; f++) pf += 3LL * 3 * (f + 1);
+   for (int N = 0; N <= p.LA; N++)
+      for (int t = N; t >= 0; t--)
+         for (int u = N - t; u >= 0; u--)
+            for (int f = p.LC0; f <= p.LC; f++)
+               for (int fx = f; fx >= 0; fx--)
+                  for (int fy = f - fx; fy >= 0; fy--) pf += 3LL * (fx + 1) * (fy + 1) * (f - fx - fy + 1);
+   for (int e = p.LA0; e <= p.LA; e++)
+      for (int ex = e; ex >= 0; ex--)
+         for (int ey = e - ex; ey >= 0; ey--) pf += 3LL * p.nkc * (ex + 1) * (ey + 1) * (e - ex - ey + 1);
+   p.prim_flops = pf;
+   tp = p;
 }
 
 }  // namespace gint

@@ -193,7 +193,7 @@ class Stream:
 
 
 def setup_handle(lib, prefix, system, hf_fac=1.0, max_mem_mib=4096, truncated=True, P=None, Pb=None,
-                 stats=False, timing=False):
+                 stats=False, timing=False, debug=0):
     """create + configure a handle of `lib` (product or oracle) for `system`"""
     f = lambda n: getattr(lib, prefix + n)
     h = f("create_handle")()
@@ -202,6 +202,8 @@ def setup_handle(lib, prefix, system, hf_fac=1.0, max_mem_mib=4096, truncated=Tr
         lib.libgint_amd_enable_stats(h, 1)
     if timing and prefix == "libgint_":
         lib.libgint_amd_enable_timing(h, 1)
+    if debug and prefix == "libgint_":
+        lib.libgint_amd_set_debug(h, debug)
     if truncated:
         c0, ld = tcg_table()
         f("set_Potential_Truncated")(h, system.rcut(), _ptr(c0.ravel()), ld, c0.size)
@@ -222,10 +224,11 @@ def setup_handle(lib, prefix, system, hf_fac=1.0, max_mem_mib=4096, truncated=Tr
     return h, table
 
 
-def kbuild(lib, prefix, system, stream, begin=0, end=None, stride=1, K0=None, **kw):
+def kbuild(lib, prefix, system, stream, begin=0, end=None, stride=1, K0=None, plan=False, **kw):
     """one K-build of (a sample of) the stream; returns K in block layout
     (as accumulated by the library, before symmetrisation) and the handle.
-    With Pb in kw the build is spin-polarised and K is (Ka, Kb)."""
+    With Pb in kw the build is spin-polarised and K is (Ka, Kb).  With
+    plan=True the stream is recorded as a resident plan and executed."""
     h, table = setup_handle(lib, prefix, system, **kw)
     polar = kw.get("Pb") is not None
     if K0 is not None:
@@ -233,7 +236,13 @@ def kbuild(lib, prefix, system, stream, begin=0, end=None, stride=1, K0=None, **
             getattr(lib, prefix + "set_K_polarized")(h, _ptr(K0[0]), _ptr(K0[1]), system.P_size)
         else:
             getattr(lib, prefix + "set_K")(h, _ptr(K0), system.P_size)
-    stream.replay(table, h, begin, end, stride)
+    if plan:
+        lib.libgint_amd_plan_begin(h)
+        stream.replay(table, h, begin, end, stride)
+        lib.libgint_amd_plan_end(h)
+        lib.libgint_amd_plan_execute(h)
+    else:
+        stream.replay(table, h, begin, end, stride)
     if polar:
         Ka = np.zeros(system.P_size)
         Kb = np.zeros(system.P_size)

@@ -539,20 +539,26 @@ static void cart_tables(void)
 static int cidx(const int v[3]) { return g_idx[v[0]][v[1]][v[2]]; }
 static int first_dir(const int v[3]) { return v[0] > 0 ? 0 : (v[1] > 0 ? 1 : 2); }
 
+/* survivor checksum key of a (slot, class): a hash of the slot (set
+   indices, images, primitive indices, cell) times an odd multiplier of the
+   class, summed mod 2^64 over the surviving pairs (order-independent) */
+#define MIX(z) do { z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL; z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL; z = z ^ (z >> 31); } while (0)
 uint64_t oracle_slot_key(int i, int j, int k, int l, int n1, int n2,
                          int la, int lb, int lc, int ld,
                          int ipa, int ipb, int ipc, int ipd, int n3)
 {
-#define MIX(z) do { z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL; z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL; z = z ^ (z >> 31); } while (0)
    uint64_t w0 = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
    uint64_t w1 = ((uint64_t)(uint32_t)k << 32) | (uint32_t)l;
-   uint64_t w2 = ((uint64_t)n1 << 24) | ((uint64_t)n2 << 16) | (uint64_t)((la << 12) | (lb << 8) | (lc << 4) | ld);
+   uint64_t w2 = ((uint64_t)n1 << 24) | ((uint64_t)n2 << 16);
    uint64_t w3 = ((uint64_t)((ipa << 12) | (ipb << 8) | (ipc << 4) | ipd) << 8) | (uint64_t)n3;
    uint64_t z = w0 + 0x9E3779B97F4A7C15ULL;
    MIX(z); z ^= w1; MIX(z); z ^= w2; MIX(z); z ^= w3; MIX(z);
-   return z;
-#undef MIX
+   int code = ((la * 4 + lb) * 4 + lc) * 4 + ld;
+   uint64_t m = 0xC1A55E5ULL + (uint64_t)code * 0x9E3779B97F4A7C15ULL;
+   MIX(m);
+   return z * (m | 1ULL);
 }
+#undef MIX
 
 /* G_m(R,T), reference src/t_c_g0_n.cpp:67-180, returns branch */
 static int o_tcg(const Oracle *o, double *res, double R, double T, int Nder, double Kfac)

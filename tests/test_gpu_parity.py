@@ -26,6 +26,8 @@ def _both(system, stream, **kw):
     o = oracle()
     Kp, hp = D.kbuild(lib, "libgint_", system, stream, stats=True, **kw)
     sp = D.stats(lib, "libgint_", hp)
+    kw.pop("plan", None)
+    kw.pop("debug", None)
     Ko, ho = D.kbuild(o, "oracle_", system, stream, **kw)
     so = D.stats(o, "oracle_", ho)
     lib.libgint_amd_destroy_handle(hp)
@@ -72,6 +74,11 @@ CASES = [
     ("water1_pbc27_d_basis4", dict(n_water=1, box_A=6.0, image_shells=1, basis=4, np_max=3, seed=2), {}),
     ("water2_open_smallmem", dict(n_water=2, box_A=5.0, image_shells=0, np_max=5, seed=9), dict(max_mem_mib=64)),
     ("water1_hf025", dict(n_water=1, box_A=4.0, image_shells=0, np_max=3, seed=4), dict(hf_fac=0.25)),
+    ("water2_pbc27_minimal", dict(n_water=2, box_A=6.0, image_shells=1, basis=1, np_max=4, seed=6), {}),
+    ("water2_pbc27_plan", dict(n_water=2, box_A=7.0, image_shells=1, np_max=3, seed=12), dict(plan=True)),
+    ("water2_pbc27_fixup", dict(n_water=2, box_A=7.0, image_shells=1, np_max=3, seed=13), dict(debug=1)),
+    ("water1_pbc125_coulomb", dict(n_water=1, box_A=6.0, image_shells=2, basis=1, np_max=3, seed=14),
+     dict(truncated=False)),
 ]
 
 
