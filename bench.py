@@ -58,7 +58,10 @@ def main():
     # the screened call stream goes through the ABI once and stays resident
     t0 = time.time()
     lib.libgint_amd_plan_begin(h)
-    st.replay(table, h)
+    nsl = 16
+    for i in range(nsl):
+        st.replay(table, h, st.n_sets * i // nsl, st.n_sets * (i + 1) // nsl)
+        print(f"# ingested {i + 1}/{nsl} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     lib.libgint_amd_plan_end(h)
     t_ing = time.time() - t0
     info = (C.c_uint64 * 6)()

@@ -265,6 +265,33 @@ __device__ __forceinline__ int class_src(const TypePlan &tp, int la, int LA, int
    return 0;
 }
 
+// sum_j T2_j sum_{l<14-j} T1_l c[jl]: the operations of gint_cheb_sum in the
+// same order (so the value is bitwise the host's), with T2_j produced by
+// the recurrence of gint_cheb14 inside the loop to keep few values live
+__device__ __forceinline__ double d_cheb_sum(const double *__restrict__ c, const double *T1, double x2)
+{
+   GINT_NO_CONTRACT
+   double tmp = 0.0, tm1 = 0.0, tm2 = 0.0;
+   int jl = 0;
+#pragma unroll 1
+   for (int j = 0; j < 14; j++) {
+      double t2;
+      if (j == 0) t2 = 1.0;
+      else if (j == 1) t2 = GINT_SQRT2 * x2;
+      else if (j == 2) t2 = 2. * x2 * tm1 - GINT_SQRT2;
+      else t2 = 2. * x2 * tm1 - tm2;
+      double dot = 0.0;
+#pragma unroll
+      for (int l = 0; l < 14; l++)
+         if (l < 14 - j) dot += T1[l] * c[jl + l];
+      tmp += dot * t2;
+      jl += 14 - j;
+      tm2 = tm1;
+      tm1 = t2;
+   }
+   return tmp;
+}
+
 // ---------------------------------------------------------------------------
 // K1: G_m -> Hermite R -> cell sum -> (e0|f0), one wavefront per primitive
 // quartet.  FIX: one wavefront per listed (primitive, cell) slot, no
@@ -272,26 +299,31 @@ __device__ __forceinline__ int class_src(const TypePlan &tp, int la, int LA, int
 // ---------------------------------------------------------------------------
 struct FmLayout {
    int SB;      // survivors per Hermite batch
-   int o_sv;    // survivor data, 5 doubles each: X Y Z of P-Q-g, TG1 (or T), TG2
+   int o_fmv;   // per lane: G_m, then D_m = (-2 rho)^m G_m
+   int o_sv;    // per lane: X Y Z of P-Q-g, TG1 (or T), TG2
    int o_R;     // Hermite layers, 2 x SB x nh
-   int o_int;   // byte offset of 64 survivor infos (branch | patch << 2)
+   int o_int;   // byte offset of 64 lane infos (branch | patch << 2) and 64 survivor lanes
    int bytes;
 };
+
+constexpr int GEO = 24;   // doubles of the primitive geometry at the start of LDS
+enum { G_P = 0, G_Q = 3, G_PA = 6, G_QC = 9, G_KFAC = 12, G_RHO = 13, G_RR = 14, G_I2AB = 15, G_I2CD = 16 };
 
 static FmLayout fm_layout(const TypePlan &tp, bool fix)
 {
    FmLayout y;
    const int L = tp.L, nh = tp.nh;
-   int SB = fix ? 1 : (3072 - 64 * (L + 1) - 320) / (2 * nh);
+   y.o_fmv = GEO;
+   y.o_sv = GEO + 64 * (L + 1);
+   y.o_R = y.o_sv + 320;
+   int SB = fix ? 1 : (3072 - y.o_R) / (2 * nh);
    SB = std::max(1, std::min(16, SB));
    y.SB = SB;
-   y.o_sv = 64 * (L + 1);
-   y.o_R = y.o_sv + 320;
    const int cell_phase = y.o_R + 2 * SB * nh;
-   const int e_phase = nh + 6 * 28 + tp.nbh * tp.nkc;
+   const int e_phase = GEO + nh + 6 * 28 + tp.nbh * tp.nkc;
    const int dbl = std::max(cell_phase, e_phase);
    y.o_int = 8 * dbl;
-   y.bytes = y.o_int + 4 * 64;
+   y.bytes = y.o_int + 4 * 128;
    return y;
 }
 
@@ -303,10 +335,12 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
    const int lane = threadIdx.x;
    const int L = tp.L, nh = tp.nh, Lp1 = L + 1;
    const int SB = lay.SB;
-   double *fmv = lds;                 // [64][L+1]: G_m, then D_m = (-2 rho)^m G_m
+   double *G = lds;
+   double *fmv = lds + lay.o_fmv;     // [64][L+1]
    double *sv = lds + lay.o_sv;       // [64][5]
    double *Rb = lds + lay.o_R;        // [2][SB][nh]
    int *svi = (int *)((char *)lds + lay.o_int);
+   int *svl = svi + 64;
    const uint32_t nwork = FIX ? min(b.fix_count[0], b.fix_cap) : (b.p1 - b.p0);
 
    for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
@@ -323,12 +357,25 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
       const GroupRec &g = b.groups[row.group];
       const uint32_t ip = b.prm_ip[p];
       const int n1 = row.n12 & 0xffff, n2 = row.n12 >> 16;
-      const double *data = b.data;
-      PrimGeo geo;
-      prim_prepare(data + g.R[0], data + g.R[1], data + g.R[2], data + g.R[3], data[g.Z[0] + (ip & 15)],
-                   data[g.Z[1] + ((ip >> 4) & 15)], data[g.Z[2] + ((ip >> 8) & 15)], data[g.Z[3] + ((ip >> 12) & 15)],
-                   n1, n2, t, geo);
-      const double m2rho = -2.0 * geo.rho;
+      if (lane == 0) {
+         const double *data = b.data;
+         PrimGeo geo;
+         prim_prepare(data + g.R[0], data + g.R[1], data + g.R[2], data + g.R[3], data[g.Z[0] + (ip & 15)],
+                      data[g.Z[1] + ((ip >> 4) & 15)], data[g.Z[2] + ((ip >> 8) & 15)],
+                      data[g.Z[3] + ((ip >> 12) & 15)], n1, n2, t, geo);
+         for (int k = 0; k < 3; k++) {
+            G[G_P + k] = geo.P[k];
+            G[G_Q + k] = geo.Q[k];
+            G[G_PA + k] = geo.PA[k];
+            G[G_QC + k] = geo.QC[k];
+         }
+         G[G_KFAC] = geo.Kfac;
+         G[G_RHO] = geo.rho;
+         G[G_RR] = geo.Rr;
+         G[G_I2AB] = geo.i2ab;
+         G[G_I2CD] = geo.i2cd;
+      }
+      __syncthreads();
       double acc[8];
 #pragma unroll
       for (int i = 0; i < 8; i++) acc[i] = 0.0;
@@ -342,152 +389,155 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
          const bool valid = FIX ? (lane == 0) : (n3 < t.Ng);
          int status = 0;  // 1 kept by every class, 2 screened by every class, 3 class-dependent
          unsigned mask = 0;
-         double X[3] = {0.0, 0.0, 0.0}, T = 0.0, Fm0 = 0.0, tg1 = 0.0, tg2 = 0.0;
-         int br = GINT_G_ZERO, patch = 0;
-         double fb[MAXLL + 1];
+         double *myf = fmv + lane * Lp1;
          if (valid) {
+            const double Kfac = G[G_KFAC], rho = G[G_RHO];
+            double X[3], T, Fm0 = 0.0, tg1 = 0.0, tg2 = 0.0;
+            int br, patch = 0;
             {
                GINT_NO_CONTRACT
-               X[0] = geo.P[0] - geo.Q[0] - t.neighs[n3 * 3 + 0];
-               X[1] = geo.P[1] - geo.Q[1] - t.neighs[n3 * 3 + 1];
-               X[2] = geo.P[2] - geo.Q[2] - t.neighs[n3 * 3 + 2];
+               X[0] = G[G_P + 0] - G[G_Q + 0] - t.neighs[n3 * 3 + 0];
+               X[1] = G[G_P + 1] - G[G_Q + 1] - t.neighs[n3 * 3 + 1];
+               X[2] = G[G_P + 2] - G[G_Q + 2] - t.neighs[n3 * 3 + 2];
                const double rpq2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
-               T = geo.rho * rpq2;
+               T = rho * rpq2;
             }
-            br = t.truncated ? gint_tcg_locate(geo.Rr, T, t.grid_lo, t.grid_hi, t.bw, &patch, &tg1, &tg2) : GINT_G_BOYS;
+            br = t.truncated ? gint_tcg_locate(G[G_RR], T, t.grid_lo, t.grid_hi, t.bw, &patch, &tg1, &tg2) : GINT_G_BOYS;
             if (br == GINT_G_PATCH) {
-               double T1[14], T2[14];
+               double T1[14];
                gint_cheb14(tg1, T1);
-               gint_cheb14(tg2, T2);
                {
                   GINT_NO_CONTRACT
-                  Fm0 = gint_cheb_sum(t.C0 + (size_t)t.ld_C0 * patch, T1, T2) * geo.Kfac;
+                  Fm0 = d_cheb_sum(t.C0 + (size_t)t.ld_C0 * patch, T1, tg2) * Kfac;
                }
+               myf[0] = Fm0;
             } else if (br == GINT_G_BOYS) {
-               gint_boys(L, T, fb, t.ftable, geo.Kfac);
-               Fm0 = fb[0];
+               gint_boys(L, T, myf, t.ftable, Kfac);
+               Fm0 = myf[0];
             }
+            const bool lsens = br == GINT_G_BOYS && T >= 1.e-13 && T < 12.;   // F_0 depends on the start order
             if (br == GINT_G_ZERO) {
                status = 2;
             } else {
                mask = Fm0 > 1.e-12 ? all : 0u;
-               // F_0 from the downward recursion depends on the starting
-               // order by rounding only; decide each class L exactly when
-               // it is that close to the threshold
-               if (br == GINT_G_BOYS && tp.nL > 1 && T >= 1.e-13 && T < 12. && fabs(Fm0 - 1.e-12) <= 1.e-20) {
+               // F_0 of the downward recursion depends on the starting order
+               // by rounding only; decide each class L exactly this close to
+               // the threshold
+               if (lsens && tp.nL > 1 && fabs(Fm0 - 1.e-12) <= 1.e-20) {
                   mask = 0u;
                   for (int i = 0; i < tp.nL; i++) {
-                     double f2[MAXLL + 1];
-                     gint_boys(tp.Ls[i], T, f2, t.ftable, geo.Kfac);
-                     if (f2[0] > 1.e-12) mask |= 1u << i;
+                     gint_boys(tp.Ls[i], T, myf, t.ftable, Kfac);
+                     if (myf[0] > 1.e-12) mask |= 1u << i;
                   }
+                  gint_boys(L, T, myf, t.ftable, Kfac);
                }
                status = mask == all ? 1 : (mask == 0u ? 2 : 3);
             }
             if (FIX) {
-               // per-L Fm[0] of the slot for the classes that screen it
+               // per-L Fm[0] of the slot, for the classes that screen it
                double *f0 = b.fix_ws + (size_t)w * (tp.ncp + 16) + tp.ncp;
                for (int i = 0; i < tp.nL; i++) {
                   double v = Fm0;
-                  if (br == GINT_G_BOYS && T >= 1.e-13 && T < 12.) {
-                     double f2[MAXLL + 1];
-                     gint_boys(tp.Ls[i], T, f2, t.ftable, geo.Kfac);
-                     v = f2[0];
+                  if (lsens) {
+                     gint_boys(tp.Ls[i], T, myf, t.ftable, Kfac);
+                     v = myf[0];
                   }
                   f0[i] = v;
                }
+               if (lsens) gint_boys(L, T, myf, t.ftable, Kfac);
                status = br == GINT_G_ZERO ? 2 : 1;
             } else if (b.force_fix && status == 1) {
                status = 3;
             }
-         }
-         if (!FIX) {
-            if (status == 2) {
-               if (tp.has00) quirk += Fm0;
-            } else if (status == 1 || status == 3) {
-               if (b.stats) {
-                  const unsigned long long h = slot_hash(g.set_ij, g.set_kl, n1, n2, ip, n3);
-                  if (mask == all) {
-                     n_surv += (unsigned long long)tp.nclass;
-                     csum += h * tp.mult_all;
-                  } else {
-                     unsigned long long cm = 0;
-                     for (int k = 0; k < tp.nclass; k++)
-                        if ((mask >> tp.cls_gate[k]) & 1u) {
-                           n_surv++;
-                           cm += tp.cls_mult[k];
-                        }
-                     csum += h * cm;
-                  }
-               }
-               if (status == 3) {
-                  const uint32_t idx = atomicAdd(&b.fix_count[0], 1u);
-                  if (idx < b.fix_cap) {
-                     b.fix[2 * idx] = p;
-                     b.fix[2 * idx + 1] = (uint32_t)n3 | (mask << 8);
-                  } else {
-                     atomicOr(&b.fix_count[1], 1u);
-                  }
+            if (!FIX) {
+               if (status == 2) {
+                  if (tp.has00) quirk += Fm0;
                } else {
-                  n_any++;
+                  if (b.stats) {
+                     const unsigned long long h = slot_hash(g.set_ij, g.set_kl, n1, n2, ip, n3);
+                     if (mask == all) {
+                        n_surv += (unsigned long long)tp.nclass;
+                        csum += h * tp.mult_all;
+                     } else {
+                        unsigned long long cm = 0;
+                        for (int k = 0; k < tp.nclass; k++)
+                           if ((mask >> tp.cls_gate[k]) & 1u) {
+                              n_surv++;
+                              cm += tp.cls_mult[k];
+                           }
+                        csum += h * cm;
+                     }
+                  }
+                  if (status == 3) {
+                     const uint32_t idx = atomicAdd(&b.fix_count[0], 1u);
+                     if (idx < b.fix_cap) {
+                        b.fix[2 * idx] = p;
+                        b.fix[2 * idx + 1] = (uint32_t)n3 | (mask << 8);
+                     } else {
+                        atomicOr(&b.fix_count[1], 1u);
+                     }
+                  } else {
+                     n_any++;
+                  }
                }
             }
+            if (status == 1) {
+               sv[lane * 5 + 0] = X[0];
+               sv[lane * 5 + 1] = X[1];
+               sv[lane * 5 + 2] = X[2];
+               sv[lane * 5 + 3] = br == GINT_G_PATCH ? tg1 : T;
+               sv[lane * 5 + 4] = tg2;
+               svi[lane] = br | (patch << 2);
+            }
          }
-         // compact the survivors of this pass
+         // survivors of this pass, in lane order
          const unsigned long long smask = __ballot(status == 1);
          const int nsv = __popcll(smask);
-         if (status == 1) {
-            const int s = __popcll(smask & ((1ull << lane) - 1ull));
-            sv[s * 5 + 0] = X[0];
-            sv[s * 5 + 1] = X[1];
-            sv[s * 5 + 2] = X[2];
-            sv[s * 5 + 3] = br == GINT_G_PATCH ? tg1 : T;
-            sv[s * 5 + 4] = tg2;
-            svi[s] = br | (patch << 2);
-            if (br == GINT_G_BOYS) {
-               for (int m = 0; m <= L; m++) fmv[s * Lp1 + m] = fb[m];
-            } else {
-               fmv[s * Lp1] = Fm0;
-            }
-         }
+         if (status == 1) svl[__popcll(smask & ((1ull << lane) - 1ull))] = lane;
          __syncthreads();
          if (nsv == 0) continue;
          // G_1..G_L of the patch survivors, one (survivor, order) per lane
-         for (int idx = lane; idx < nsv * L; idx += 64) {
-            const int s = idx / L, m = idx - s * L + 1;
-            const int info = svi[s];
-            if ((info & 3) == GINT_G_PATCH) {
-               double T1[14], T2[14];
-               gint_cheb14(sv[s * 5 + 3], T1);
-               gint_cheb14(sv[s * 5 + 4], T2);
-               const double *c = t.C0 + (size_t)t.ld_C0 * (info >> 2) + m * TCG_NCOEF;
-               fmv[s * Lp1 + m] = gint_cheb_sum(c, T1, T2) * geo.Kfac;
+         {
+            const double Kfac = G[G_KFAC];
+            for (int idx = lane; idx < nsv * L; idx += 64) {
+               const int s = idx / L, m = idx - s * L + 1;
+               const int ln = svl[s];
+               const int info = svi[ln];
+               if ((info & 3) == GINT_G_PATCH) {
+                  double T1[14];
+                  gint_cheb14(sv[ln * 5 + 3], T1);
+                  const double *c = t.C0 + (size_t)t.ld_C0 * (info >> 2) + m * TCG_NCOEF;
+                  fmv[ln * Lp1 + m] = d_cheb_sum(c, T1, sv[ln * 5 + 4]) * Kfac;
+               }
             }
          }
          __syncthreads();
-         for (int idx = lane; idx < nsv * Lp1; idx += 64) {
-            const int m = idx % Lp1;
-            double pw = 1.0;
-            for (int k = 0; k < m; k++) pw *= m2rho;
-            fmv[idx] *= pw;
+         {
+            const double m2rho = -2.0 * G[G_RHO];
+            for (int idx = lane; idx < nsv * Lp1; idx += 64) {
+               const int s = idx / Lp1, m = idx - s * Lp1;
+               double pw = 1.0;
+               for (int k = 0; k < m; k++) pw *= m2rho;
+               fmv[svl[s] * Lp1 + m] *= pw;
+            }
          }
          __syncthreads();
          // Hermite recursion R^(n)_tuv = coef R^(n+1)_{tuv-2d} + X_d R^(n+1)_{tuv-d}
          for (int s0 = 0; s0 < nsv; s0 += SB) {
             const int nb = min(SB, nsv - s0);
             double *prv = Rb, *cur = Rb + SB * nh;
-            if (lane < nb) prv[lane * nh] = fmv[(s0 + lane) * Lp1 + L];
+            if (lane < nb) prv[lane * nh] = fmv[svl[s0 + lane] * Lp1 + L];
             __syncthreads();
             for (int n = L - 1; n >= 1; n--) {
                const int mn = d_nherm(L - n);
                for (int h = lane; h < mn; h += 64) {
                   if (h == 0) {
-                     for (int s = 0; s < nb; s++) cur[s * nh] = fmv[(s0 + s) * Lp1 + n];
+                     for (int s = 0; s < nb; s++) cur[s * nh] = fmv[svl[s0 + s] * Lp1 + n];
                   } else {
                      const HermEntry e = c_herm.e[h];
                      const double cf = (double)e.coef;
                      for (int s = 0; s < nb; s++) {
-                        double v = sv[(s0 + s) * 5 + e.dir] * prv[s * nh + e.m1];
+                        double v = sv[svl[s0 + s] * 5 + e.dir] * prv[s * nh + e.m1];
                         if (e.m2 >= 0) v += cf * prv[s * nh + e.m2];
                         cur[s * nh + h] = v;
                      }
@@ -505,12 +555,12 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
                if (h < nh) {
                   double a = acc[i];
                   if (h == 0) {
-                     for (int s = 0; s < nb; s++) a += fmv[(s0 + s) * Lp1];
+                     for (int s = 0; s < nb; s++) a += fmv[svl[s0 + s] * Lp1];
                   } else {
                      const HermEntry e = c_herm.e[h];
                      const double cf = (double)e.coef;
                      for (int s = 0; s < nb; s++) {
-                        double v = sv[(s0 + s) * 5 + e.dir] * prv[s * nh + e.m1];
+                        double v = sv[svl[s0 + s] * 5 + e.dir] * prv[s * nh + e.m1];
                         if (e.m2 >= 0) v += cf * prv[s * nh + e.m2];
                         a += v;
                      }
@@ -523,7 +573,7 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
       }
 
       // ---- Hermite -> cartesian, once per primitive quartet ----
-      double *sR = lds;                  // [nh]
+      double *sR = lds + GEO;            // [nh]
       double *Ex = sR + nh;              // [3][28] bra E^{e0}_t
       double *Fk = Ex + 84;              // [3][28] ket (-1)^tau E^{f0}_tau
       double *W = Fk + 84;               // [nbh][nkc]
@@ -539,8 +589,8 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
          const int d = lane % 3;
          const bool ket = lane >= 3;
          const int Lm = ket ? tp.LC : tp.LA;
-         const double x = ket ? geo.QC[d] : geo.PA[d];
-         const double i2 = ket ? geo.i2cd : geo.i2ab;
+         const double x = ket ? G[G_QC + d] : G[G_PA + d];
+         const double i2 = ket ? G[G_I2CD] : G[G_I2AB];
          double *E = (ket ? Fk : Ex) + d * 28;
          E[0] = 1.0;
          for (int e = 0; e < Lm; e++) {
