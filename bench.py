@@ -91,12 +91,19 @@ def main():
     tm = (C.c_double * 12)()
     lib.libgint_amd_timing(h, tm)
     tot = sum(times)
+    for cat, nm in ((0, "fm_r by type L"), (2, "gqs by type L")):
+        det = (C.c_double * 256)()
+        lib.libgint_amd_timing_detail(h, cat, det)
+        top = sorted(((det[i] / a.steps, i) for i in range(256) if det[i] > 0), reverse=True)[:12]
+        lab = lambda i: f"L{i}"
+        print(f"# {nm}: " + ", ".join(f"{lab(i)} {ms:.1f}ms" for ms, i in top), file=sys.stderr, flush=True)
     slots = stats[1] / a.steps          # surviving (slot, class) pairs per K-build
     value = slots / (tot / a.steps)
-    names = ["fm_r", "eco", "hrr_sph_ks", "fixup"]
-    kms = [tm[i] / a.steps for i in range(4)]
-    kfl = [stats[6] / a.steps, tm[9] / a.steps, tm[10] / a.steps, 0.0]
-    nl = [tm[4 + i] / a.steps for i in range(4)]
+    names = ["fm_r", "gqs", "fixup"]
+    cat = [0, 2, 3]
+    kms = [tm[i] / a.steps for i in cat]
+    kfl = [stats[6] / a.steps, tm[8 + 2] / a.steps, 0.0]
+    nl = [tm[4 + i] / a.steps for i in cat]
     dom = int(np.argmax(kms))
     # average duration of one launch of the dominant kernel, and its FLOPs per launch
     per_launch_ms = kms[dom] / max(nl[dom], 1)

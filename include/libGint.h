@@ -100,11 +100,14 @@ void libgint_amd_enable_stats(void *handle, int on);
 void libgint_amd_stats(void *handle, uint64_t out[8]);
 /* per-kernel device time with HIP events on the handle's stream */
 void libgint_amd_enable_timing(void *handle, int on);
-/* out[0..3] ms in fm_r / eco / hrr_sph_ks / fix-up, out[4..7] launches,
-   out[9], out[10] algorithmic FLOPs of eco and hrr_sph_ks (out[8] = 0: the
-   fm_r FLOPs are libgint_amd_stats out[6]; out[11] = 0) */
+/* out[0..3] ms in fm_r / (unused) / the quartet kernel gqs / fix-up,
+   out[4..7] launches, out[10] algorithmic FLOPs of gqs (out[8], out[9],
+   out[11] = 0: the fm_r FLOPs are libgint_amd_stats out[6]) */
 void libgint_amd_timing(void *handle, double out[12]);
 void libgint_amd_reset_counters(void *handle);
+/* ms per launch key of one category of libgint_amd_timing: fm_r (0) and the
+   quartet kernel (2) by the quartet type's total angular momentum */
+void libgint_amd_timing_detail(void *handle, int cat, double out[256]);
 /* bit 0: route every surviving slot through the class-dependent screening
    path (a test of that path; results are unchanged) */
 void libgint_amd_set_debug(void *handle, int flags);

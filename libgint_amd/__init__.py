@@ -54,6 +54,7 @@ ABI = {
     "libgint_amd_enable_timing": (None, [_vp, _i]),
     "libgint_amd_timing": (None, [_vp, _dp]),
     "libgint_amd_reset_counters": (None, [_vp]),
+    "libgint_amd_timing_detail": (None, [_vp, _i, _dp]),
     "libgint_amd_set_debug": (None, [_vp, _i]),
     "libgint_amd_plan_begin": (None, [_vp]),
     "libgint_amd_plan_end": (None, [_vp]),

@@ -113,6 +113,7 @@ void libgint_amd_timing(void *h, double out[12])
    }
 }
 void libgint_amd_reset_counters(void *h) { E(h)->reset_counters(); }
+void libgint_amd_timing_detail(void *h, int cat, double out[256]) { E(h)->timing_detail(cat, out); }
 void libgint_amd_set_debug(void *h, int flags) { E(h)->set_debug(flags); }
 void libgint_amd_plan_begin(void *h) { E(h)->plan_begin(); }
 void libgint_amd_plan_end(void *h) { E(h)->plan_end(); }

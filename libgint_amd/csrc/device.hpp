@@ -12,6 +12,8 @@
 #include <stdint.h>
 #include <stddef.h>
 
+#include <vector>
+
 namespace gint {
 
 constexpr int MAXL = 3;          // highest angular momentum of a shell (f)
@@ -121,6 +123,8 @@ struct TypePlan {
    int cls_nl[MAXTC];        // nla*nlb*nlc*nld
    int cls_ncp[MAXTC];
    int cls_eoff[MAXTC];      // offset of class k inside a row's contracted doubles
+   int co_off[4][MAXSH];     // quartet kernel: coefficients of shell s of centre t in LDS, nl x 16
+   int nco_tot;
    unsigned long long cls_mult[MAXTC];   // checksum multiplier of class k (odd)
    unsigned long long mult_all;          // sum of cls_mult
    // algorithmic FLOPs of this library's kernels (Hermite path)
@@ -148,6 +152,8 @@ struct DevTables {
 // from the chunk's first primitive and first row.
 struct BatchDev {
    const TypePlan *tp;        // device copy
+   const uint32_t *e1;        // Hermite -> cartesian stage 1 items: t u v fx fy fz, 4 bits each
+   const uint32_t *e2;        // stage 2 items: ex ey ez (4 bits each), column << 12
    const RowRec *rows;
    const GroupRec *groups;
    const uint32_t *prm_row;   // row of each primitive quartet
@@ -156,11 +162,10 @@ struct BatchDev {
    const KsRec *ksx;
    const double *data;
    double *ac0;               // [primitive - p0][ncp]
-   double *eco;               // [row - r0][ebytes]
    uint32_t p0, p1;           // primitives of the chunk
    uint32_t r0, r1;           // rows
    uint32_t g0, g1;           // groups
-   unsigned long long *stats; // [0] surviving (slot, class) pairs, [1] checksum, [2] K1 FLOPs (nullable)
+   unsigned long long *stats; // STATS_SLOTS x {surviving (slot, class) pairs, checksum, K1 FLOPs} (nullable)
    uint32_t *fix;             // mixed-gate slots: (primitive, n3 | mask << 8)
    uint32_t *fix_count;       // [0] entries, [1] overflow
    uint32_t fix_cap;
@@ -181,20 +186,37 @@ struct Scratch {
 };
 
 constexpr unsigned SCRATCH_GRID = 2048;
+constexpr unsigned STATS_SLOTS = 4096;   // counter triples the kernels spread their statistics over
+
+// LDS plan of one class in the quartet kernel
+struct CqLayout {
+   int lb;          // contractions per pass
+   int lws;         // workspace doubles per contraction: 3 ycap + Nc
+   int o_ws;        // workspace in LDS (lb x lws), unless ws_global
+   int o_kp;        // coefficient products [lb][CQ_PCH]
+   int o_I;         // spherical integrals of every contraction [nq][Ns] (reduce)
+   int o_src;       // first entry of the class's block map (class element -> type element) in the type's maps
+   int reduce;
+   int ws_global;   // workspace in HBM (gws + blockIdx * lb * lws)
+   int bytes;
+};
 
 // launchers (kernels.hip), asynchronous on `stream`; 0 or a negative code
 int launch_fm_r(const TypePlan &tp, const DevTables &t, const BatchDev &b, void *stream);
-int launch_eco(const TypePlan &tp, const BatchDev &b, void *stream);
 int launch_fixup(const TypePlan &tp, const DevTables &t, const BatchDev &b, uint32_t n_fix, void *stream);
-int launch_hrr_sph_ks(const ClassPlan &cp, int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b,
-                      const KTargets &kt, Scratch s, void *stream);
-size_t scratch_doubles(const ClassPlan &cp);
+// per-class LDS plans and block maps of a type for the quartet kernel
+void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::vector<CqLayout> &lays,
+                  std::vector<int> &srcmaps, int *lds_bytes, int *gws_stride);
+int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lays, const int *d_srcmaps, int lds_bytes,
+               int gws_stride, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s, void *stream);
 // LDS bytes of the fused kernel for a type (0 if it does not fit)
 size_t fm_r_lds_bytes(const TypePlan &tp);
 
 ClassPlan make_plan(int la, int lb, int lc, int ld);
 // fills everything from nsh, l and nl
 void make_type_plan(TypePlan &tp);
+// item descriptors of the Hermite -> cartesian transform of a type
+void make_type_items(const TypePlan &tp, std::vector<uint32_t> &e1, std::vector<uint32_t> &e2);
 uint64_t class_mult(int code);
 void boys_table(double *ftable);
 int device_math_selftest(const double *x, const double *y, int n, double *out);

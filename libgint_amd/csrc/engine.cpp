@@ -133,8 +133,14 @@ Engine::~Engine()
    if (stream_) (void)hipStreamSynchronize((hipStream_t)stream_);
    for (auto &d : stream_dev_) d.release();
    for (auto &d : plan_dev_) d.release();
-   for (auto &ti : types_)
+   for (auto &ti : types_) {
       if (ti.d_plan) (void)hipFree(ti.d_plan);
+      if (ti.d_e1) (void)hipFree(ti.d_e1);
+      if (ti.d_e2) (void)hipFree(ti.d_e2);
+      if (ti.d_cps) (void)hipFree(ti.d_cps);
+      if (ti.d_lays) (void)hipFree(ti.d_lays);
+      if (ti.d_srcmaps) (void)hipFree(ti.d_srcmaps);
+   }
    void *bufs[] = {d_data_, d_C0_, d_grid_, d_bw_, d_ftab_, d_cell_, d_neighs_, d_work_,
                    d_fix_, d_fix_ws_, d_scratch_, d_stats_, d_fixcnt_};
    for (void *b : bufs)
@@ -251,11 +257,13 @@ int Engine::type_of(int i, int j, int k, int l)
          }
       }
       make_type_plan(ti.plan);
+      make_type_items(ti.plan, ti.e1, ti.e2);
       if (!fm_r_lds_bytes(ti.plan)) fatal("quartet type exceeds the on-chip budget", ti.plan.L);
       for (int c = 0; c < ti.plan.nclass; c++) {
          const int code = ti.plan.cls_code[c];
          ti.cplans.push_back(make_plan(code / 64, (code / 16) % 4, (code / 4) % 4, code % 4));
       }
+      make_cq_plan(ti.plan, ti.cplans, ti.lays, ti.srcmaps, &ti.cq_lds, &ti.cq_gws);
       types_.push_back(ti);
       q_.emplace_back();
       it = type_ids_.emplace(sig, (int)types_.size() - 1).first;
@@ -695,6 +703,16 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
       if (!ti.d_plan) {
          HIPCHECK(hipMalloc((void **)&ti.d_plan, sizeof(TypePlan)));
          HIPCHECK(hipMemcpy(ti.d_plan, &tp, sizeof(TypePlan), hipMemcpyHostToDevice));
+         HIPCHECK(hipMalloc((void **)&ti.d_e1, sizeof(uint32_t) * ti.e1.size()));
+         HIPCHECK(hipMemcpy(ti.d_e1, ti.e1.data(), sizeof(uint32_t) * ti.e1.size(), hipMemcpyHostToDevice));
+         HIPCHECK(hipMalloc((void **)&ti.d_e2, sizeof(uint32_t) * ti.e2.size()));
+         HIPCHECK(hipMemcpy(ti.d_e2, ti.e2.data(), sizeof(uint32_t) * ti.e2.size(), hipMemcpyHostToDevice));
+         HIPCHECK(hipMalloc((void **)&ti.d_cps, sizeof(ClassPlan) * ti.cplans.size()));
+         HIPCHECK(hipMemcpy(ti.d_cps, ti.cplans.data(), sizeof(ClassPlan) * ti.cplans.size(), hipMemcpyHostToDevice));
+         HIPCHECK(hipMalloc((void **)&ti.d_lays, sizeof(CqLayout) * ti.lays.size()));
+         HIPCHECK(hipMemcpy(ti.d_lays, ti.lays.data(), sizeof(CqLayout) * ti.lays.size(), hipMemcpyHostToDevice));
+         HIPCHECK(hipMalloc((void **)&ti.d_srcmaps, sizeof(int) * ti.srcmaps.size()));
+         HIPCHECK(hipMemcpy(ti.d_srcmaps, ti.srcmaps.data(), sizeof(int) * ti.srcmaps.size(), hipMemcpyHostToDevice));
       }
       dev_put(D.rows, D.cap[0], Q.rows);
       dev_put(D.groups, D.cap[1], Q.groups);
@@ -710,15 +728,14 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
       for (size_t gi = 0; gi < D.n_groups; gi++)
          for (int k = 0; k < tp.nclass; k++)
             if (ks_count(Q.ksg[gi * tp.nclass + k].flags)) D.class_used[k] = 1;
-      // chunks of whole groups
+      // chunks of whole groups whose primitive blocks fit the work buffer
       Chunk c{};
       size_t bytes = 0;
-      const size_t row_b = sizeof(double) * (size_t)tp.ebytes;
       for (size_t gi = 0; gi < D.n_groups; gi++) {
          const GroupRec &g = Q.groups[gi];
          size_t gp = 0;
          for (uint32_t r = g.row0; r < g.row1; r++) gp += Q.rows[r].n_prm;
-         const size_t gb = gp * sizeof(double) * (size_t)tp.ncp + (size_t)(g.row1 - g.row0) * row_b;
+         const size_t gb = gp * sizeof(double) * (size_t)tp.ncp;
          if (gb > work) fatal("one set quartet needs more device memory than set_max_mem allows (MiB)", (long long)(gb >> 20));
          if (bytes + gb > work && c.g1 > c.g0) {
             D.chunks.push_back(c);
@@ -737,12 +754,12 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
          c.r1 = g.row1;
          c.p1 = (uint32_t)(Q.rows[g.row0].prm_off + gp);
          bytes += gb;
-         // algorithmic FLOPs of the contraction and of the quartet kernel
-         c.eco_flops += 2.0 * (double)gp * tp.ebytes;
+         // algorithmic FLOPs of the quartet kernel: contraction, HRR, SPH, KS
          for (int k = 0; k < tp.nclass; k++) {
             const double nq = ks_count(Q.ksg[gi * tp.nclass + k].flags);
             const ClassPlan &cp = ti.cplans[k];
-            c.k3_flops += nq * ((double)(g.row1 - g.row0) * cp.hrr_flops + cp.sph_flops + 9.0 * cp.Ns);
+            c.k3_flops += nq * (2.0 * (double)gp * cp.ncp + (double)(g.row1 - g.row0) * cp.hrr_flops + cp.sph_flops +
+                                9.0 * cp.Ns);
          }
       }
       if (c.g1 > c.g0) D.chunks.push_back(c);
@@ -761,8 +778,8 @@ void Engine::ensure_work(size_t ncp_max, size_t max_prm_chunk)
       work_bytes_ = want;
    }
    if (!d_stats_) {
-      HIPCHECK(hipMalloc((void **)&d_stats_, 4 * sizeof(unsigned long long)));
-      HIPCHECK(hipMemset(d_stats_, 0, 4 * sizeof(unsigned long long)));
+      HIPCHECK(hipMalloc((void **)&d_stats_, 3 * STATS_SLOTS * sizeof(unsigned long long)));
+      HIPCHECK(hipMemset(d_stats_, 0, 3 * STATS_SLOTS * sizeof(unsigned long long)));
       HIPCHECK(hipMalloc((void **)&d_fixcnt_, 2 * sizeof(uint32_t)));
       HIPCHECK(hipMemset(d_fixcnt_, 0, 2 * sizeof(uint32_t)));
    }
@@ -787,6 +804,7 @@ void Engine::ev_begin(int cat)
 {
    (void)cat;
    if (!timing_on_) return;
+   ev_key_ = 0;
    if (ev_used_ + 2 > ev_pool_.size()) {
       hipEvent_t e0, e1;
       HIPCHECK(hipEventCreate(&e0));
@@ -803,7 +821,7 @@ void Engine::ev_end(int cat)
    if (!timing_on_) return;
    void *e1 = ev_pool_[ev_used_ + 1];
    HIPCHECK(hipEventRecord((hipEvent_t)e1, (hipStream_t)stream_));
-   ev_live_.push_back(EvRec{cat, ev_open_, e1, 0.0});
+   ev_live_.push_back(EvRec{cat, ev_open_, e1, ev_key_});
    ev_used_ += 2;
 }
 
@@ -814,6 +832,7 @@ void Engine::harvest_events()
       HIPCHECK(hipEventElapsedTime(&ms, (hipEvent_t)r.e0, (hipEvent_t)r.e1));
       timing_.ms[r.cat] += ms;
       timing_.launches[r.cat]++;
+      detail_[r.cat][r.key & 255] += ms;
    }
    ev_live_.clear();
    ev_used_ = 0;
@@ -838,6 +857,8 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
 
    BatchDev b;
    b.tp = ti.d_plan;
+   b.e1 = ti.d_e1;
+   b.e2 = ti.d_e2;
    b.rows = td.rows;
    b.groups = td.groups;
    b.prm_row = td.prm_row;
@@ -846,7 +867,6 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
    b.ksx = td.ksx;
    b.data = d_data_;
    b.ac0 = d_work_;
-   b.eco = d_work_ + (size_t)(c.p1 - c.p0) * tp.ncp;
    b.p0 = c.p0;
    b.p1 = c.p1;
    b.r0 = c.r0;
@@ -870,26 +890,20 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
 
    HIPCHECK(hipMemsetAsync(d_fixcnt_, 0, sizeof(uint32_t), st));
    ev_begin(0);
+   ev_key_ = tp.L;
    int rc = launch_fm_r(tp, t, b, st);
    if (rc) fatal("fm_r launch", rc);
    ev_end(0);
-   ev_begin(1);
-   rc = launch_eco(tp, b, st);
-   if (rc) fatal("eco launch", rc);
-   ev_end(1);
    ev_begin(3);
    rc = launch_fixup(tp, t, b, (debug_ & 1) ? (uint32_t)fix_cap_ : 64u, st);
    if (rc) fatal("fix-up launch", rc);
    ev_end(3);
    ev_begin(2);
-   for (int k = 0; k < tp.nclass; k++) {
-      if (!td.class_used[k]) continue;
-      const ClassPlan &cp = ti.cplans[k];
-      rc = launch_hrr_sph_ks(cp, k, tp, t, b, kt, Scratch{d_scratch_, scratch_cap_}, st);
-      if (rc) fatal("hrr_sph_ks launch", rc);
-   }
+   ev_key_ = tp.L;
+   rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, t, b, kt, Scratch{d_scratch_, scratch_cap_},
+                   st);
+   if (rc) fatal("gqs launch", rc);
    ev_end(2);
-   timing_.flops[1] += c.eco_flops;
    timing_.flops[2] += c.k3_flops;
    n_prmc_ += (uint64_t)(c.p1 - c.p0) * tp.nclass;
    n_slots_ += (uint64_t)(c.p1 - c.p0) * (uint64_t)t.Ng * tp.nclass;
@@ -906,7 +920,7 @@ void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
       const TypeInfo &ti = types_[D.type];
       ncp_max = std::max(ncp_max, (size_t)ti.plan.ncp);
       for (auto &c : D.chunks) max_prm = std::max(max_prm, (size_t)(c.p1 - c.p0));
-      for (auto &cp : ti.cplans) scr = std::max(scr, scratch_doubles(cp));
+      scr = std::max(scr, (size_t)ti.cq_gws * SCRATCH_GRID);
    }
    if (ncp_max == 0) return;
    ensure_work(ncp_max, max_prm);
@@ -928,7 +942,8 @@ void Engine::reset_counters()
    synchronize();
    n_slots_ = n_prmc_ = n_q_ = n_rows_ = 0;
    timing_ = Timing();
-   if (d_stats_) HIPCHECK(hipMemset(d_stats_, 0, 4 * sizeof(unsigned long long)));
+   if (d_stats_) HIPCHECK(hipMemset(d_stats_, 0, 3 * STATS_SLOTS * sizeof(unsigned long long)));
+   for (auto &d : detail_) d.fill(0.0);
 }
 
 Timing Engine::timing()
@@ -937,11 +952,22 @@ Timing Engine::timing()
    return timing_;
 }
 
+void Engine::timing_detail(int cat, double out[256])
+{
+   synchronize();
+   for (int i = 0; i < 256; i++) out[i] = (cat >= 0 && cat < 4) ? detail_[cat][i] : 0.0;
+}
+
 void Engine::stats(uint64_t out[8])
 {
    synchronize();
-   unsigned long long s[4] = {0, 0, 0, 0};
-   if (d_stats_) HIPCHECK(hipMemcpy(s, d_stats_, sizeof s, hipMemcpyDeviceToHost));
+   unsigned long long s[3] = {0, 0, 0};
+   if (d_stats_) {
+      std::vector<unsigned long long> part(3 * STATS_SLOTS);
+      HIPCHECK(hipMemcpy(part.data(), d_stats_, sizeof(unsigned long long) * part.size(), hipMemcpyDeviceToHost));
+      for (unsigned i = 0; i < STATS_SLOTS; i++)
+         for (int k = 0; k < 3; k++) s[k] += part[3 * i + k];
+   }
    out[0] = n_slots_;
    out[1] = s[0];
    out[2] = n_prmc_;

@@ -59,7 +59,7 @@ struct TypeQueue {
 
 struct Chunk {
    uint32_t g0, g1, r0, r1, p0, p1;
-   double eco_flops, k3_flops;
+   double k3_flops;
 };
 
 // one type's work on the device
@@ -81,13 +81,21 @@ struct TypeInfo {
    std::array<int, 4> sig;
    TypePlan plan;
    TypePlan *d_plan = nullptr;
+   std::vector<uint32_t> e1, e2;          // Hermite -> cartesian item descriptors
+   uint32_t *d_e1 = nullptr, *d_e2 = nullptr;
    std::vector<ClassPlan> cplans;
+   std::vector<CqLayout> lays;            // quartet kernel: per-class LDS plans
+   std::vector<int> srcmaps;              // class block element -> type block element
+   int cq_lds = 0, cq_gws = 0;
+   ClassPlan *d_cps = nullptr;
+   CqLayout *d_lays = nullptr;
+   int *d_srcmaps = nullptr;
 };
 
 struct Timing {
-   double ms[4] = {0, 0, 0, 0};        // fm_r, eco, hrr_sph_ks, fixup
+   double ms[4] = {0, 0, 0, 0};        // fm_r, (unused), cqs, fixup
    uint64_t launches[4] = {0, 0, 0, 0};
-   double flops[4] = {0, 0, 0, 0};     // host-counted algorithmic FLOPs of eco and hrr_sph_ks
+   double flops[4] = {0, 0, 0, 0};     // host-counted algorithmic FLOPs of cqs in [2]
 };
 
 class Engine {
@@ -123,6 +131,7 @@ class Engine {
    void stats(uint64_t out[8]);
    void enable_timing(bool on) { timing_on_ = on; }
    Timing timing();
+   void timing_detail(int cat, double out[256]);
    void reset_counters();
    void set_debug(int flags) { debug_ = flags; }
    void plan_begin();
@@ -214,8 +223,10 @@ class Engine {
    struct EvRec {
       int cat;
       void *e0, *e1;
-      double flops;
+      int key;
    };
+   int ev_key_ = 0;
+   std::array<std::array<double, 256>, 4> detail_{};   // ms per category and key (type L / class code)
    std::vector<void *> ev_pool_;
    std::vector<EvRec> ev_live_;
    size_t ev_used_ = 0;

@@ -12,17 +12,17 @@
 //                  classes at once, reference src/compute_Fm.cpp (prepare,
 //                  Fm, Vm), src/compute_VRR.cpp and the cell sum of
 //                  src/compute_ECO.cpp:216-272.
-//   k_eco          one workgroup per row: contraction over primitives for
-//                  every class, reference src/compute_ECO.cpp:12-214.
-//   k_fm_r<FIX>,   slots whose screening differs between the classes of the
-//   k_fix_eco      type (Fm[0] within rounding of 1e-12 at different L): the
-//                  slot's blocks alone, added to the classes that keep it.
-//   k_hrr_sph_ks   one wavefront per (set quartet, class): HRR of each
-//                  (n1,n2) row summed on chip, cartesian -> spherical, and
-//                  the exchange update of K, reduced over the set quartet's
-//                  contractions before the atomics.  Reference
-//                  src/compute_HRR.cpp, src/compute_SPH.cpp:58-237,
-//                  src/compute_KS.cpp:8-117.
+//   k_fm_r<FIX>    slots whose screening differs between the classes of the
+//                  type (Fm[0] within rounding of 1e-12 at different L): the
+//                  slot's blocks alone, added by k_cqs to the classes that
+//                  keep it.
+//   k_cqs          one workgroup per (set quartet, class), a batch of its
+//                  contractions at once: contraction over primitives
+//                  (src/compute_ECO.cpp:12-214), HRR of each (n1,n2) row
+//                  (src/compute_HRR.cpp) summed on chip, cartesian ->
+//                  spherical (src/compute_SPH.cpp:58-237), and the exchange
+//                  update of K (src/compute_KS.cpp:8-117) reduced over the
+//                  set quartet's contractions before the atomics.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -250,6 +250,17 @@ __device__ __forceinline__ double kprod(const TypePlan &tp, const GroupRec &g, c
           data[g.Kof[2][sc] + ic * npc + ((ip >> 8) & 15)] * data[g.Kof[3][sd] + id * npd + ((ip >> 12) & 15)];
 }
 
+// the same product from the coefficients staged in LDS (layout co_off)
+__device__ __forceinline__ double kprod_lds(const TypePlan &tp, const double *co, int k, int l, uint32_t ip)
+{
+   const int sh = tp.cls_sh[k];
+   const int sa = sh & 15, sb = (sh >> 4) & 15, sc = (sh >> 8) & 15, sd = (sh >> 12) & 15;
+   const int nlb = tp.nl[1][sb], nlc = tp.nl[2][sc], nld = tp.nl[3][sd];
+   const int ia = l / (nlb * nlc * nld), ib = (l / (nlc * nld)) % nlb, ic = (l / nld) % nlc, id = l % nld;
+   return co[tp.co_off[0][sa] + ia * MAXNP + (ip & 15)] * co[tp.co_off[1][sb] + ib * MAXNP + ((ip >> 4) & 15)] *
+          co[tp.co_off[2][sc] + ic * MAXNP + ((ip >> 8) & 15)] * co[tp.co_off[3][sd] + id * MAXNP + ((ip >> 12) & 15)];
+}
+
 // offset in the type's block set of element j of a class's block set
 // (blocks e in [la,LA], f in [lc,LC], e outer, layout [ie][jf] in a block)
 __device__ __forceinline__ int class_src(const TypePlan &tp, int la, int LA, int lc, int LC, int j)
@@ -273,7 +284,7 @@ __device__ __forceinline__ double d_cheb_sum(const double *__restrict__ c, const
    GINT_NO_CONTRACT
    double tmp = 0.0, tm1 = 0.0, tm2 = 0.0;
    int jl = 0;
-#pragma unroll 1
+#pragma unroll 7
    for (int j = 0; j < 14; j++) {
       double t2;
       if (j == 0) t2 = 1.0;
@@ -316,7 +327,7 @@ static FmLayout fm_layout(const TypePlan &tp, bool fix)
    y.o_fmv = GEO;
    y.o_sv = GEO + 64 * (L + 1);
    y.o_R = y.o_sv + 320;
-   int SB = fix ? 1 : (3072 - y.o_R) / (2 * nh);
+   int SB = fix ? 1 : (1792 - y.o_R) / (2 * nh);
    SB = std::max(1, std::min(16, SB));
    y.SB = SB;
    const int cell_phase = y.o_R + 2 * SB * nh;
@@ -381,6 +392,7 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
       for (int i = 0; i < 8; i++) acc[i] = 0.0;
       double quirk = 0.0;
       unsigned long long n_surv = 0, csum = 0, n_any = 0;
+      bool any_surv = false;
       const unsigned all = (1u << tp.nL) - 1u;
       const int ncell = FIX ? 1 : t.Ng;
 
@@ -496,6 +508,7 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
          if (status == 1) svl[__popcll(smask & ((1ull << lane) - 1ull))] = lane;
          __syncthreads();
          if (nsv == 0) continue;
+         any_surv = true;
          // G_1..G_L of the patch survivors, one (survivor, order) per lane
          {
             const double Kfac = G[G_KFAC];
@@ -577,82 +590,86 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
       double *Ex = sR + nh;              // [3][28] bra E^{e0}_t
       double *Fk = Ex + 84;              // [3][28] ket (-1)^tau E^{f0}_tau
       double *W = Fk + 84;               // [nbh][nkc]
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < 8; i++) {
-         const int h = lane + 64 * i;
-         if (h < nh) sR[h] = acc[i];
-      }
       for (int off = 32; off > 0; off >>= 1) quirk += __shfl_down(quirk, off, 64);
       quirk = __shfl(quirk, 0, 64);
-      if (lane < 6) {
-         const int d = lane % 3;
-         const bool ket = lane >= 3;
-         const int Lm = ket ? tp.LC : tp.LA;
-         const double x = ket ? G[G_QC + d] : G[G_PA + d];
-         const double i2 = ket ? G[G_I2CD] : G[G_I2AB];
-         double *E = (ket ? Fk : Ex) + d * 28;
-         E[0] = 1.0;
-         for (int e = 0; e < Lm; e++) {
-            const double *Ee = E + e * (e + 1) / 2;
-            double *En = E + (e + 1) * (e + 2) / 2;
-            for (int tt = 0; tt <= e + 1; tt++) {
-               double v = 0.0;
-               if (tt > 0) v += i2 * Ee[tt - 1];
-               if (tt <= e) v += x * Ee[tt];
-               if (tt + 1 <= e) v += (double)(tt + 1) * Ee[tt + 1];
-               En[tt] = v;
-            }
-         }
-         if (ket)
-            for (int e = 0; e <= Lm; e++)
-               for (int tt = 1; tt <= e; tt += 2) E[e * (e + 1) / 2 + tt] = -E[e * (e + 1) / 2 + tt];
-      }
-      __syncthreads();
-      const int nkc = tp.nkc;
-      for (int idx = lane; idx < tp.nbh * nkc; idx += 64) {
-         const int hb = idx / nkc, jf = idx - hb * nkc;
-         const int tt = c_herm.tuv[hb][0], uu = c_herm.tuv[hb][1], vv = c_herm.tuv[hb][2];
-         int f = tp.LC0;
-         while (jf >= tp.kcoff[f] + d_nco(f)) f++;
-         const int ic = jf - tp.kcoff[f];
-         const int fx = c_cart.comp[f][ic][0], fy = c_cart.comp[f][ic][1], fz = c_cart.comp[f][ic][2];
-         const double *Fx = Fk + fx * (fx + 1) / 2, *Fy = Fk + 28 + fy * (fy + 1) / 2, *Fz = Fk + 56 + fz * (fz + 1) / 2;
-         double s = 0.0;
-         for (int a = 0; a <= fx; a++)
-            for (int c = 0; c <= fy; c++) {
-               const double cxy = Fx[a] * Fy[c];
-               for (int d = 0; d <= fz; d++) s += cxy * Fz[d] * sR[c_herm.idx[tt + a][uu + c][vv + d]];
-            }
-         W[idx] = s;
-      }
-      __syncthreads();
       double *dst = FIX ? b.fix_ws + (size_t)w * (tp.ncp + 16) : b.ac0 + (size_t)(p - b.p0) * tp.ncp;
-      for (int j = lane; j < tp.ncp; j += 64) {
-         int rem = j, e = tp.LA0;
-         while (rem >= d_nco(e) * nkc) {
-            rem -= d_nco(e) * nkc;
-            e++;
+      if (!FIX && !any_surv) {
+         // no cell survives: the blocks are zero but for the screened slots'
+         // Fm[0] in (00|00)
+         for (int j = lane; j < tp.ncp; j += 64) dst[j] = (j == 0 && tp.has00) ? quirk : 0.0;
+         __syncthreads();
+      } else {
+         __syncthreads();
+#pragma unroll
+         for (int i = 0; i < 8; i++) {
+            const int h = lane + 64 * i;
+            if (h < nh) sR[h] = acc[i];
          }
-         int f = tp.LC0;
-         while (rem >= d_nco(e) * d_nco(f)) {
-            rem -= d_nco(e) * d_nco(f);
-            f++;
-         }
-         const int ie = rem / d_nco(f), jl = rem - ie * d_nco(f);
-         const int ex = c_cart.comp[e][ie][0], ey = c_cart.comp[e][ie][1], ez = c_cart.comp[e][ie][2];
-         const double *Ea = Ex + ex * (ex + 1) / 2, *Eb = Ex + 28 + ey * (ey + 1) / 2, *Ec = Ex + 56 + ez * (ez + 1) / 2;
-         const int col = tp.kcoff[f] + jl;
-         double s = 0.0;
-         for (int a = 0; a <= ex; a++)
-            for (int c = 0; c <= ey; c++) {
-               const double cxy = Ea[a] * Eb[c];
-               for (int d = 0; d <= ez; d++) s += cxy * Ec[d] * W[c_herm.idx[a][c][d] * nkc + col];
+         if (lane < 6) {
+            const int d = lane % 3;
+            const bool ket = lane >= 3;
+            const int Lm = ket ? tp.LC : tp.LA;
+            const double x = ket ? G[G_QC + d] : G[G_PA + d];
+            const double i2 = ket ? G[G_I2CD] : G[G_I2AB];
+            double *E = (ket ? Fk : Ex) + d * 28;
+            E[0] = 1.0;
+            for (int e = 0; e < Lm; e++) {
+               const double *Ee = E + e * (e + 1) / 2;
+               double *En = E + (e + 1) * (e + 2) / 2;
+               for (int tt = 0; tt <= e + 1; tt++) {
+                  double v = 0.0;
+                  if (tt > 0) v += i2 * Ee[tt - 1];
+                  if (tt <= e) v += x * Ee[tt];
+                  if (tt + 1 <= e) v += (double)(tt + 1) * Ee[tt + 1];
+                  En[tt] = v;
+               }
             }
-         if (!FIX && j == 0 && tp.has00) s += quirk;
-         dst[j] = s;
+            if (ket)
+               for (int e = 0; e <= Lm; e++)
+                  for (int tt = 1; tt <= e; tt += 2) E[e * (e + 1) / 2 + tt] = -E[e * (e + 1) / 2 + tt];
+         }
+         __syncthreads();
+         const int nkc = tp.nkc;
+         // W[tuv][f] = sum E^f_tau E^f_nu E^f_phi R[t+tau, u+nu, v+phi]
+         for (int idx = lane; idx < tp.nbh * nkc; idx += 64) {
+            const uint32_t it = b.e1[idx];
+            const int tt = it & 15, uu = (it >> 4) & 15, vv = (it >> 8) & 15;
+            const int fx = (it >> 12) & 15, fy = (it >> 16) & 15, fz = (it >> 20) & 15;
+            const double *Fx = Fk + fx * (fx + 1) / 2, *Fy = Fk + 28 + fy * (fy + 1) / 2, *Fz = Fk + 56 + fz * (fz + 1) / 2;
+            double s = 0.0;
+            for (int a = 0; a <= fx; a++)
+               for (int c = 0; c <= fy; c++) {
+                  const double cxy = Fx[a] * Fy[c];
+                  const int t1 = tt + a, u1 = uu + c;
+                  for (int d = 0; d <= fz; d++) {
+                     const int v1 = vv + d, N = t1 + u1 + v1;
+                     const int H = N * (N + 1) * (N + 2) / 6 + (N - t1) * (N - t1 + 1) / 2 + v1;
+                     s += cxy * Fz[d] * sR[H];
+                  }
+               }
+            W[idx] = s;
+         }
+         __syncthreads();
+         // (e0|f0) = sum E^e_t E^e_u E^e_v W[tuv][f]
+         for (int j = lane; j < tp.ncp; j += 64) {
+            const uint32_t it = b.e2[j];
+            const int ex = it & 15, ey = (it >> 4) & 15, ez = (it >> 8) & 15, col = (int)(it >> 12);
+            const double *Ea = Ex + ex * (ex + 1) / 2, *Eb = Ex + 28 + ey * (ey + 1) / 2, *Ec = Ex + 56 + ez * (ez + 1) / 2;
+            double s = 0.0;
+            for (int a = 0; a <= ex; a++)
+               for (int c = 0; c <= ey; c++) {
+                  const double cxy = Ea[a] * Eb[c];
+                  for (int d = 0; d <= ez; d++) {
+                     const int N = a + c + d;
+                     const int H = N * (N + 1) * (N + 2) / 6 + (N - a) * (N - a + 1) / 2 + d;
+                     s += cxy * Ec[d] * W[H * nkc + col];
+                  }
+               }
+            if (!FIX && j == 0 && tp.has00) s += quirk;
+            dst[j] = s;
+         }
+         __syncthreads();
       }
-      __syncthreads();
 
       if (!FIX && b.stats) {
          for (int off = 32; off > 0; off >>= 1) {
@@ -661,124 +678,48 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
             n_any += __shfl_down(n_any, off, 64);
          }
          if (lane == 0) {
+            // per-workgroup partials spread over STATS_SLOTS counters: one
+            // address hit by every workgroup of the chip serialises them
+            unsigned long long *st = b.stats + 3 * (blockIdx.x & (STATS_SLOTS - 1));
             if (n_surv) {
-               atomicAdd(&b.stats[0], n_surv);
-               atomicAdd(&b.stats[1], csum);
+               atomicAdd(&st[0], n_surv);
+               atomicAdd(&st[1], csum);
             }
-            atomicAdd(&b.stats[2], n_any * (unsigned long long)tp.surv_flops + (unsigned long long)tp.prim_flops);
+            atomicAdd(&st[2], n_any * (unsigned long long)tp.surv_flops + (unsigned long long)tp.prim_flops);
          }
       }
    }
 }
 
 // ---------------------------------------------------------------------------
-// fix-up: add a listed slot's blocks to the classes that keep it and its
-// Fm[0] (the quirk of reference src/compute_VRR.cpp:384-385) to the
-// classes with la = lc = 0 that screen it
+// K2: per (set quartet, class), for a batch of its contractions at once:
+// contraction over each row's primitive quartets (reference
+// src/compute_ECO.cpp:12-214), HRR of the row (src/compute_HRR.cpp), sum
+// over the rows, cartesian -> spherical (src/compute_SPH.cpp:58-237) and
+// the exchange update of K (src/compute_KS.cpp:8-117), reduced over the set
+// quartet's contractions before the atomics when the spherical integrals of
+// all contractions fit on chip.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_fix_eco(const TypePlan *__restrict__ tpp, BatchDev b)
+constexpr int CQ_THREADS = 128;
+constexpr int CPL_BYTES = (sizeof(ClassPlan) + 15) & ~15;   // LDS copy of the class plan
+constexpr int CQ_PCH = 16;   // primitive quartets per contraction step
+
+
+__device__ __forceinline__ double *lds_base()
 {
-   const TypePlan &tp = *tpp;
-   const uint32_t n = min(b.fix_count[0], b.fix_cap);
-   for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
-      const uint32_t p = b.fix[2 * w];
-      const unsigned mask = b.fix[2 * w + 1] >> 8;
-      const uint32_t r = b.prm_row[p];
-      const RowRec row = b.rows[r];
-      const GroupRec &g = b.groups[row.group];
-      const uint32_t ip = b.prm_ip[p];
-      const double *src = b.fix_ws + (size_t)w * (tp.ncp + 16);
-      double *erow = b.eco + (size_t)(r - b.r0) * tp.ebytes;
-      for (int k = 0; k < tp.nclass; k++) {
-         const int sh = tp.cls_sh[k];
-         const int la = tp.l[0][sh & 15], lb = tp.l[1][(sh >> 4) & 15], lc = tp.l[2][(sh >> 8) & 15],
-                   ld = tp.l[3][(sh >> 12) & 15];
-         const int nq = tp.cls_nl[k], ncpk = tp.cls_ncp[k];
-         double *out = erow + tp.cls_eoff[k];
-         if ((mask >> tp.cls_gate[k]) & 1u) {
-            for (int idx = threadIdx.x; idx < nq * ncpk; idx += blockDim.x) {
-               const int l = idx / ncpk, j = idx - l * ncpk;
-               const double v = src[class_src(tp, la, la + lb, lc, lc + ld, j)];
-               unsafeAtomicAdd(&out[idx], kprod(tp, g, b.data, k, l, ip) * v);
-            }
-         } else if (la == 0 && lc == 0) {
-            for (int l = threadIdx.x; l < nq; l += blockDim.x)
-               unsafeAtomicAdd(&out[l * ncpk], kprod(tp, g, b.data, k, l, ip) * src[tp.ncp + tp.cls_gate[k]]);
-         }
-      }
-   }
+   extern __shared__ double lds[];
+   return lds;
 }
 
-// ---------------------------------------------------------------------------
-// K2: contraction over primitives, one 256-thread workgroup per row, all
-// classes of the type.  eco[row][eoff_k + l ncp_k + j] =
-//    sum_p Ka Kb Kc Kd (p, l) ac0[p][class_src(j)]
-// ---------------------------------------------------------------------------
-constexpr int ECO_THREADS = 256;
-constexpr int ECO_PCHUNK = 16;
-constexpr int ECO_PER_THREAD = 4;
-
-__global__ __launch_bounds__(ECO_THREADS) void k_eco(const TypePlan *__restrict__ tpp, BatchDev b)
+// x / d for 0 <= x < 2^22, d >= 1: float reciprocal estimate, one correction
+__device__ __forceinline__ int fdiv(int x, int d)
 {
-   __shared__ double sK[MAXNL * MAXNL * MAXNL * MAXNL * ECO_PCHUNK];
-   const TypePlan &tp = *tpp;
-   const int tid = threadIdx.x;
-   const uint32_t r = b.r0 + blockIdx.x;
-   const RowRec row = b.rows[r];
-   const GroupRec &g = b.groups[row.group];
-   const int ncp = tp.ncp;
-   double *erow = b.eco + (size_t)(r - b.r0) * tp.ebytes;
-   const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * ncp;
-   const uint32_t *IP = b.prm_ip + row.prm_off;
-   const int nprm = (int)row.n_prm;
-   for (int k = 0; k < tp.nclass; k++) {
-      const int sh = tp.cls_sh[k];
-      const int la = tp.l[0][sh & 15], lb = tp.l[1][(sh >> 4) & 15], lc = tp.l[2][(sh >> 8) & 15],
-                ld = tp.l[3][(sh >> 12) & 15];
-      const int nq = tp.cls_nl[k], ncpk = tp.cls_ncp[k];
-      const int total = nq * ncpk;
-      for (int tile = 0; tile < total; tile += ECO_THREADS * ECO_PER_THREAD) {
-         double accv[ECO_PER_THREAD];
-         int src[ECO_PER_THREAD], lq[ECO_PER_THREAD];
-#pragma unroll
-         for (int u = 0; u < ECO_PER_THREAD; u++) {
-            accv[u] = 0.0;
-            const int idx = tile + u * ECO_THREADS + tid;
-            lq[u] = idx / ncpk;
-            src[u] = idx < total ? class_src(tp, la, la + lb, lc, lc + ld, idx - lq[u] * ncpk) : 0;
-         }
-         for (int p0 = 0; p0 < nprm; p0 += ECO_PCHUNK) {
-            const int np = min(ECO_PCHUNK, nprm - p0);
-            __syncthreads();
-            for (int idx = tid; idx < nq * ECO_PCHUNK; idx += ECO_THREADS) {
-               const int l = idx / ECO_PCHUNK, pp = idx - l * ECO_PCHUNK;
-               sK[idx] = pp < np ? kprod(tp, g, b.data, k, l, IP[p0 + pp]) : 0.0;
-            }
-            __syncthreads();
-            const double *A = A0 + (size_t)p0 * ncp;
-#pragma unroll
-            for (int u = 0; u < ECO_PER_THREAD; u++) {
-               const int idx = tile + u * ECO_THREADS + tid;
-               if (idx < total) {
-                  double s = accv[u];
-                  for (int pp = 0; pp < np; pp++) s += sK[lq[u] * ECO_PCHUNK + pp] * A[(size_t)pp * ncp + src[u]];
-                  accv[u] = s;
-               }
-            }
-         }
-#pragma unroll
-         for (int u = 0; u < ECO_PER_THREAD; u++) {
-            const int idx = tile + u * ECO_THREADS + tid;
-            if (idx < total) erow[tp.cls_eoff[k] + idx] = accv[u];
-         }
-      }
-   }
+   int q = (int)((float)x * __builtin_amdgcn_rcpf((float)d));
+   const int rem = x - q * d;
+   if (rem < 0) q--;
+   else if (rem >= d) q++;
+   return q;
 }
-
-// ---------------------------------------------------------------------------
-// K3: per (set quartet, class): HRR over its rows, sum, cart->sph, K update
-// ---------------------------------------------------------------------------
-constexpr int Q_THREADS = 64;
 
 __device__ void row_ab_cd(const GroupRec &g, const RowRec &row, const double *data, const DevTables &t, double AB[3],
                           double CD[3])
@@ -811,237 +752,344 @@ __device__ __forceinline__ unsigned blk_idx(bool T, unsigned ld, unsigned row_i,
    return T ? row_i * ld + col_i : col_i * ld + row_i;
 }
 
-__global__ __launch_bounds__(Q_THREADS) void k_hrr_sph_ks(ClassPlan cp, int cls_k, const TypePlan *__restrict__ tpp,
-                                                          DevTables t, BatchDev b, KTargets kt, double *gws, int icap)
+// one (set quartet, class): the work of k_gqs for class cls_k of group gi
+__device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b,
+                       const KTargets &kt, const CqLayout &y, double *WS, const int *__restrict__ srcmap, uint32_t gi,
+                       const double *co, uint32_t *ipb)
 {
    extern __shared__ double lds[];
-   const TypePlan &tp = *tpp;
    const int tid = threadIdx.x;
-   const int cap = cp.ycap;
-   double *ws = gws ? gws + (size_t)blockIdx.x * cp.hrr_ws : lds;
-   double *Iall = gws ? lds : lds + cp.hrr_ws;
-   double *buf[3] = {ws, ws + cap, ws + 2 * cap};
-   double *acc = ws + 3 * cap;
+   const int cap = cp.ycap, lws = y.lws;
+   double *Kp = lds + y.o_kp;
+   double *Iall = lds + y.o_I;
    const int la = cp.la, lb = cp.lb, lc = cp.lc, ld = cp.ld, LC = cp.LC;
-   const int Nc = cp.Nc, Ns = cp.Ns;
+   const int Nc = cp.Nc, Ns = cp.Ns, ncp = cp.ncp, ncpt = tp.ncp;
    const int ncd = d_nco(lc) * d_nco(ld);
    const int sh = tp.cls_sh[cls_k];
    const int nla = tp.nl[0][sh & 15], nlb = tp.nl[1][(sh >> 4) & 15], nlc = tp.nl[2][(sh >> 8) & 15],
              nld = tp.nl[3][(sh >> 12) & 15];
    const int nq_all = nla * nlb * nlc * nld;
-   const int eoff = tp.cls_eoff[cls_k];
    const int nsa = 2 * la + 1, nsb = 2 * lb + 1, nsc = 2 * lc + 1, nsd = 2 * ld + 1;
    const int nscd = nsc * nsd, nsbcd = nsb * nscd;
    const double symm[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
-
-   for (uint32_t gi = b.g0 + blockIdx.x; gi < b.g1; gi += gridDim.x) {
-      const KsGroup ks = b.ksg[(size_t)gi * tp.nclass + cls_k];
-      const int nq = (int)(ks.flags >> 16);
-      if (nq == 0) continue;
-      const GroupRec &g = b.groups[gi];
-      const bool expl = (ks.flags & KS_EXPLICIT) != 0;
-      const bool reduce = !expl && nq == nq_all && nq * Ns <= icap;
-      for (int l = 0; l < nq; l++) {
-         __syncthreads();
-         for (int i = tid; i < Nc; i += Q_THREADS) acc[i] = 0.0;
-         for (uint32_t r = g.row0; r < g.row1; r++) {
-            const RowRec row = b.rows[r];
-            double AB[3], CD[3];
-            row_ab_cd(g, row, b.data, t, AB, CD);
-            const double *E = b.eco + (size_t)(r - b.r0) * tp.ebytes + eoff + (size_t)l * cp.ncp;
+   const uint32_t nfix = min(b.fix_count[0], b.fix_cap);
+   const int gate = tp.cls_gate[cls_k];
+   const KsGroup ks = b.ksg[(size_t)gi * tp.nclass + cls_k];
+   const int nq = (int)(ks.flags >> 16);
+   if (nq == 0) return;
+   const GroupRec &g = b.groups[gi];
+   const bool expl = (ks.flags & KS_EXPLICIT) != 0;
+   const bool reduce = y.reduce && !expl && nq == nq_all;
+   for (int l0 = 0; l0 < nq; l0 += y.lb) {
+      const int nb = min(y.lb, nq - l0);
+      __syncthreads();
+      for (int idx = tid; idx < nb * Nc; idx += CQ_THREADS) {
+         const int l = fdiv(idx, Nc);
+         WS[l * lws + 3 * cap + (idx - l * Nc)] = 0.0;
+      }
+      for (uint32_t r = g.row0; r < g.row1; r++) {
+         const RowRec row = b.rows[r];
+         // ---- contraction of the row's primitive quartets into buf0 ----
+         for (int idx = tid; idx < nb * ncp; idx += CQ_THREADS) {
+            const int l = fdiv(idx, ncp);
+            WS[l * lws + (idx - l * ncp)] = 0.0;
+         }
+         const uint32_t *IP = b.prm_ip + row.prm_off;
+         const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * ncpt;
+         for (int pc = 0; pc < (int)row.n_prm; pc += CQ_PCH) {
+            const int np = min(CQ_PCH, (int)row.n_prm - pc);
             __syncthreads();
-            for (int i = tid; i < cp.ncp; i += Q_THREADS) buf[0][i] = E[i];
+            if (tid < np) ipb[tid] = IP[pc + tid];
             __syncthreads();
-            int src = 0;
-            // ket HRR (d -> c): layer dd from layer dd-1, blocks (e,c) e outer
-            for (int dd = 1; dd <= ld; dd++) {
-               const int dst = (src + 1) % 3;
-               const int nd = d_nco(dd), ndm = d_nco(dd - 1);
-               for (int idx0 = tid; idx0 < cp.ksize[dd]; idx0 += Q_THREADS) {
-                  int e = la, c = lc;
-                  for (;;) {
-                     const int end_ec = cp.koff[dd][e][c] + d_nco(e) * d_nco(c) * nd;
-                     if (idx0 < end_ec) break;
-                     if (++c > LC - dd) {
-                        c = lc;
-                        e++;
-                     }
+            for (int idx = tid; idx < nb * CQ_PCH; idx += CQ_THREADS) {
+               const int l = idx / CQ_PCH, pp = idx % CQ_PCH;
+               Kp[idx] = pp < np ? kprod_lds(tp, co, cls_k, l0 + l, ipb[pp]) : 0.0;
+            }
+            __syncthreads();
+            const double *A = A0 + (size_t)pc * ncpt;
+            for (int idx = tid; idx < nb * ncp; idx += CQ_THREADS) {
+               const int l = fdiv(idx, ncp), j = idx - l * ncp;
+               const int src = srcmap[j];
+               const double *kp = Kp + l * CQ_PCH;
+               double s = 0.0;
+#pragma unroll
+               for (int pp = 0; pp < CQ_PCH; pp++)
+                  if (pp < np) s += kp[pp] * A[(size_t)pp * ncpt + src];
+               WS[l * lws + j] += s;
+            }
+         }
+         // slots of this row whose screening depends on the class
+         if (nfix) {
+            __syncthreads();
+            for (uint32_t e = 0; e < nfix; e++) {
+               const uint32_t p = b.fix[2 * e];
+               if (b.prm_row[p] != r) continue;
+               const bool keep = ((b.fix[2 * e + 1] >> 8) >> gate) & 1u;
+               const double *src = b.fix_ws + (size_t)e * (ncpt + 16);
+               const uint32_t ip = b.prm_ip[p];
+               if (keep) {
+                  for (int idx = tid; idx < nb * ncp; idx += CQ_THREADS) {
+                     const int l = fdiv(idx, ncp), j = idx - l * ncp;
+                     WS[l * lws + j] += kprod(tp, g, b.data, cls_k, l0 + l, ip) * src[srcmap[j]];
                   }
-                  const int idx = idx0 - cp.koff[dd][e][c];
-                  const int ncc = d_nco(c), ncc1 = d_nco(c + 1);
-                  const int ie = idx / (ncc * nd);
-                  const int rem = idx - ie * ncc * nd;
-                  const int ic = rem / nd, id = rem - ic * nd;
-                  const int dir = c_cart.first[dd][id];
-                  const int idm = c_cart.minus[dd][id][dir];
-                  const int icp = c_cart.plus[c][ic][dir];
-                  const double v1 = buf[src][cp.koff[dd - 1][e][c + 1] + (ie * ncc1 + icp) * ndm + idm];
-                  const double v0 = buf[src][cp.koff[dd - 1][e][c] + (ie * ncc + ic) * ndm + idm];
-                  buf[dst][idx0] = v1 + CD[dir] * v0;
+               } else if (la == 0 && lc == 0) {
+                  // a screened slot keeps its Fm[0] (reference src/compute_VRR.cpp:384-385)
+                  for (int l = tid; l < nb; l += CQ_THREADS)
+                     WS[l * lws] += kprod(tp, g, b.data, cls_k, l0 + l, ip) * src[ncpt + gate];
                }
                __syncthreads();
-               src = dst;
             }
-            // bra HRR (b -> a): layer bb from layer bb-1, blocks a in [la, LA-bb]
-            for (int bb = 1; bb <= lb; bb++) {
-               const int dst = (src + 1) % 3;
-               const int nb = d_nco(bb), nbm = d_nco(bb - 1);
-               for (int idx0 = tid; idx0 < cp.bsize[bb]; idx0 += Q_THREADS) {
-                  int a = la;
-                  while (idx0 >= cp.boff[bb][a] + d_nco(a) * nb * ncd) a++;
-                  const int idx = idx0 - cp.boff[bb][a];
-                  const int ia = idx / (nb * ncd);
-                  const int rem = idx - ia * nb * ncd;
-                  const int ib = rem / ncd, icd = rem - ib * ncd;
-                  const int dir = c_cart.first[bb][ib];
-                  const int ibm = c_cart.minus[bb][ib][dir];
-                  const int iap = c_cart.plus[a][ia][dir];
-                  const double v1 = buf[src][cp.boff[bb - 1][a + 1] + (iap * nbm + ibm) * ncd + icd];
-                  const double v0 = buf[src][cp.boff[bb - 1][a] + (ia * nbm + ibm) * ncd + icd];
-                  buf[dst][idx0] = v1 + AB[dir] * v0;
-               }
-               __syncthreads();
-               src = dst;
-            }
-            // the first block of the last layer is (la lb|lc ld)
-            for (int i = tid; i < Nc; i += Q_THREADS) acc[i] += buf[src][i];
          }
          __syncthreads();
-         // cartesian -> spherical: four stages, each moves the leading index last
-         {
-            const int ls[4] = {la, lb, lc, ld};
-            int dims[4] = {d_nco(la), d_nco(lb), d_nco(lc), d_nco(ld)};
-            const double *in = acc;
-            for (int s = 0; s < 4; s++) {
-               double *o = buf[s & 1];
-               const int k = dims[0];
-               const int n = dims[1] * dims[2] * dims[3];
-               const int m = 2 * ls[s] + 1;
-               const double *c2s = c_c2s + c_c2s_off[ls[s]];
-               for (int idx = tid; idx < n * m; idx += Q_THREADS) {
-                  const int j = idx / m, i = idx - j * m;
-                  double v = 0.0;
-                  for (int qq = 0; qq < k; qq++) v += c2s[i * k + qq] * in[qq * n + j];
-                  o[idx] = v;
+         double AB[3], CD[3];
+         row_ab_cd(g, row, b.data, t, AB, CD);
+         int srcb = 0;
+         // ket HRR (d -> c): layer dd from layer dd-1, blocks (e,c) e outer
+         for (int dd = 1; dd <= ld; dd++) {
+            const int dst = (srcb + 1) % 3;
+            const int nd = d_nco(dd), ndm = d_nco(dd - 1);
+            const int sz = cp.ksize[dd];
+            for (int idx1 = tid; idx1 < nb * sz; idx1 += CQ_THREADS) {
+               const int l = fdiv(idx1, sz), idx0 = idx1 - l * sz;
+               const double *in = WS + l * lws + srcb * cap;
+               int e = la, c = lc;
+               for (;;) {
+                  const int end_ec = cp.koff[dd][e][c] + d_nco(e) * d_nco(c) * nd;
+                  if (idx0 < end_ec) break;
+                  if (++c > LC - dd) {
+                     c = lc;
+                     e++;
+                  }
                }
-               __syncthreads();
-               in = o;
-               const int nd0 = dims[1], nd1 = dims[2], nd2 = dims[3];
-               dims[0] = nd0;
-               dims[1] = nd1;
-               dims[2] = nd2;
-               dims[3] = m;
+               const int idx = idx0 - cp.koff[dd][e][c];
+               const int ncc = d_nco(c), ncc1 = d_nco(c + 1);
+               const int ie = idx / (ncc * nd);
+               const int rem = idx - ie * ncc * nd;
+               const int ic = rem / nd, id = rem - ic * nd;
+               const int dir = c_cart.first[dd][id];
+               const int idm = c_cart.minus[dd][id][dir];
+               const int icp = c_cart.plus[c][ic][dir];
+               const double v1 = in[cp.koff[dd - 1][e][c + 1] + (ie * ncc1 + icp) * ndm + idm];
+               const double v0 = in[cp.koff[dd - 1][e][c] + (ie * ncc + ic) * ndm + idm];
+               WS[l * lws + dst * cap + idx0] = v1 + CD[dir] * v0;
             }
+            __syncthreads();
+            srcb = dst;
          }
-         const double *I = buf[1];
-         if (reduce) {
-            for (int i = tid; i < Ns; i += Q_THREADS) Iall[l * Ns + i] = I[i];
-            continue;
+         // bra HRR (b -> a): layer bb from layer bb-1, blocks a in [la, LA-bb]
+         for (int bb = 1; bb <= lb; bb++) {
+            const int dst = (srcb + 1) % 3;
+            const int nbb = d_nco(bb), nbm = d_nco(bb - 1);
+            const int sz = cp.bsize[bb];
+            for (int idx1 = tid; idx1 < nb * sz; idx1 += CQ_THREADS) {
+               const int l = fdiv(idx1, sz), idx0 = idx1 - l * sz;
+               const double *in = WS + l * lws + srcb * cap;
+               int a = la;
+               while (idx0 >= cp.boff[bb][a] + d_nco(a) * nbb * ncd) a++;
+               const int idx = idx0 - cp.boff[bb][a];
+               const int ia = idx / (nbb * ncd);
+               const int rem = idx - ia * nbb * ncd;
+               const int ib = rem / ncd, icd = rem - ib * ncd;
+               const int dir = c_cart.first[bb][ib];
+               const int ibm = c_cart.minus[bb][ib][dir];
+               const int iap = c_cart.plus[a][ia][dir];
+               const double v1 = in[cp.boff[bb - 1][a + 1] + (iap * nbm + ibm) * ncd + icd];
+               const double v0 = in[cp.boff[bb - 1][a] + (ia * nbm + ibm) * ncd + icd];
+               WS[l * lws + dst * cap + idx0] = v1 + AB[dir] * v0;
+            }
+            __syncthreads();
+            srcb = dst;
          }
-         // one quartet: 4 atomic updates per spherical integral, reference
-         // src/compute_KS.cpp:51-115
+         // the first block of the last layer is (la lb|lc ld)
+         for (int idx = tid; idx < nb * Nc; idx += CQ_THREADS) {
+            const int l = fdiv(idx, Nc), i = idx - l * Nc;
+            WS[l * lws + 3 * cap + i] += WS[l * lws + srcb * cap + i];
+         }
+         __syncthreads();
+      }
+      // cartesian -> spherical: four stages, each moves the leading index last
+      {
+         const int ls[4] = {la, lb, lc, ld};
+         int dims[4] = {d_nco(la), d_nco(lb), d_nco(lc), d_nco(ld)};
+         int in_off = 3 * cap;
+         for (int s = 0; s < 4; s++) {
+            const int out_off = (s & 1) * cap;
+            const int k = dims[0];
+            const int n = dims[1] * dims[2] * dims[3];
+            const int m = 2 * ls[s] + 1;
+            const double *c2s = c_c2s + c_c2s_off[ls[s]];
+            for (int idx1 = tid; idx1 < nb * n * m; idx1 += CQ_THREADS) {
+               const int l = fdiv(idx1, n * m), idx = idx1 - l * n * m;
+               const int j = idx / m, i = idx - j * m;
+               const double *in = WS + l * lws + in_off;
+               double v = 0.0;
+               for (int qq = 0; qq < k; qq++) v += c2s[i * k + qq] * in[qq * n + j];
+               WS[l * lws + out_off + idx] = v;
+            }
+            __syncthreads();
+            in_off = out_off;
+            const int nd0 = dims[1], nd1 = dims[2], nd2 = dims[3];
+            dims[0] = nd0;
+            dims[1] = nd1;
+            dims[2] = nd2;
+            dims[3] = m;
+         }
+      }
+      // spherical integrals of contraction l0+l at WS + l lws + cap
+      if (reduce) {
+         for (int idx = tid; idx < nb * Ns; idx += CQ_THREADS) {
+            const int l = fdiv(idx, Ns);
+            Iall[(l0 + l) * Ns + (idx - l * Ns)] = WS[l * lws + cap + (idx - l * Ns)];
+         }
+         continue;
+      }
+      // one contraction at a time: 4 atomic updates per spherical integral
+      for (int idx = tid; idx < nb * Ns; idx += CQ_THREADS) {
+         const int l = fdiv(idx, Ns), tt = idx - l * Ns;
          size_t off[4];
          uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
          if (expl) {
-            const KsRec x = b.ksx[ks.expl + l];
+            const KsRec x = b.ksx[ks.expl + l0 + l];
             for (int q = 0; q < 4; q++) off[q] = x.off[q];
             ld01 = x.ld01;
             ld23 = x.ld23;
             flags = x.flags;
          } else {
-            ks_offsets(ks, nlb, nlc, nld, l, off);
+            ks_offsets(ks, nlb, nlc, nld, l0 + l, off);
          }
          const double fac = symm[(flags >> 4) & 7] * kt.hf_fac;
          const unsigned ldac = ld01 & 0xffff, ldad = ld01 >> 16, ldbc = ld23 & 0xffff, ldbd = ld23 >> 16;
-         for (int tt = tid; tt < Ns; tt += Q_THREADS) {
-            const unsigned isa = tt / nsbcd, isb = (tt / nscd) % nsb, isc = (tt / nsd) % nsc, isd = tt % nsd;
-            const size_t Iac = off[0] + blk_idx(flags & 1, ldac, isa, isc);
-            const size_t Iad = off[1] + blk_idx(flags & 2, ldad, isa, isd);
-            const size_t Ibc = off[2] + blk_idx(flags & 4, ldbc, isb, isc);
-            const size_t Ibd = off[3] + blk_idx(flags & 8, ldbd, isb, isd);
-            const double iabcd = -fac * I[tt];
-            for (int s = 0; s < kt.nspin; s++) {
-               const double *P = kt.P[s];
-               double *K = kt.K[s];
-               unsafeAtomicAdd(&K[Ibd], iabcd * P[Iac]);
-               unsafeAtomicAdd(&K[Ibc], iabcd * P[Iad]);
-               unsafeAtomicAdd(&K[Iad], iabcd * P[Ibc]);
-               unsafeAtomicAdd(&K[Iac], iabcd * P[Ibd]);
+         const unsigned isa = tt / nsbcd, isb = (tt / nscd) % nsb, isc = (tt / nsd) % nsc, isd = tt % nsd;
+         const size_t Iac = off[0] + blk_idx(flags & 1, ldac, isa, isc);
+         const size_t Iad = off[1] + blk_idx(flags & 2, ldad, isa, isd);
+         const size_t Ibc = off[2] + blk_idx(flags & 4, ldbc, isb, isc);
+         const size_t Ibd = off[3] + blk_idx(flags & 8, ldbd, isb, isd);
+         const double iabcd = -fac * WS[l * lws + cap + tt];
+         for (int sp = 0; sp < kt.nspin; sp++) {
+            const double *P = kt.P[sp];
+            double *K = kt.K[sp];
+            unsafeAtomicAdd(&K[Ibd], iabcd * P[Iac]);
+            unsafeAtomicAdd(&K[Ibc], iabcd * P[Iad]);
+            unsafeAtomicAdd(&K[Iad], iabcd * P[Ibc]);
+            unsafeAtomicAdd(&K[Iac], iabcd * P[Ibd]);
+         }
+      }
+   }
+   if (!reduce) return;
+   __syncthreads();
+   // the same updates, summed over the set quartet's contractions first:
+   // K_bd += i P_ac, K_bc += i P_ad, K_ad += i P_bc, K_ac += i P_bd
+   const double fac = symm[(ks.flags >> 4) & 7] * kt.hf_fac;
+   if (fac == 0.0) return;
+   const unsigned ldac = ks.ld01 & 0xffff, ldad = ks.ld01 >> 16, ldbc = ks.ld23 & 0xffff, ldbd = ks.ld23 >> 16;
+   const bool Tac = ks.flags & 1, Tad = ks.flags & 2, Tbc = ks.flags & 4, Tbd = ks.flags & 8;
+   const int NA = nla * nsa, NB = nlb * nsb, NC = nlc * nsc, ND = nld * nsd;
+   auto Iat = [&](int ia, int isa, int ib, int isb, int ic, int isc, int id, int isd) {
+      return Iall[(((ia * nlb + ib) * nlc + ic) * nld + id) * Ns + ((isa * nsb + isb) * nsc + isc) * nsd + isd];
+   };
+   auto offs = [&](int ia, int ib, int ic, int id, int q) {
+      switch (q) {
+         case 0: return (long long)ks.base[0] + (long long)ia * ks.step[0] + (long long)ic * ks.step[1];
+         case 1: return (long long)ks.base[1] + (long long)ia * ks.step[2] + (long long)id * ks.step[3];
+         case 2: return (long long)ks.base[2] + (long long)ib * ks.step[4] + (long long)ic * ks.step[5];
+         default: return (long long)ks.base[3] + (long long)ib * ks.step[6] + (long long)id * ks.step[7];
+      }
+   };
+   for (int sp = 0; sp < kt.nspin; sp++) {
+      const double *P = kt.P[sp];
+      double *K = kt.K[sp];
+      for (int x = tid; x < NB * ND; x += CQ_THREADS) {  // K_bd
+         const int ib = (x / ND) / nsb, isb = (x / ND) % nsb, id = (x % ND) / nsd, isd = (x % ND) % nsd;
+         double sum = 0.0;
+         for (int ia = 0; ia < nla; ia++)
+            for (int ic = 0; ic < nlc; ic++) {
+               const size_t o = (size_t)offs(ia, 0, ic, 0, 0);
+               for (int isa = 0; isa < nsa; isa++)
+                  for (int isc = 0; isc < nsc; isc++)
+                     sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tac, ldac, isa, isc)];
+            }
+         unsafeAtomicAdd(&K[(size_t)offs(0, ib, 0, id, 3) + blk_idx(Tbd, ldbd, isb, isd)], -fac * sum);
+      }
+      for (int x = tid; x < NB * NC; x += CQ_THREADS) {  // K_bc
+         const int ib = (x / NC) / nsb, isb = (x / NC) % nsb, ic = (x % NC) / nsc, isc = (x % NC) % nsc;
+         double sum = 0.0;
+         for (int ia = 0; ia < nla; ia++)
+            for (int id = 0; id < nld; id++) {
+               const size_t o = (size_t)offs(ia, 0, 0, id, 1);
+               for (int isa = 0; isa < nsa; isa++)
+                  for (int isd = 0; isd < nsd; isd++)
+                     sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tad, ldad, isa, isd)];
+            }
+         unsafeAtomicAdd(&K[(size_t)offs(0, ib, ic, 0, 2) + blk_idx(Tbc, ldbc, isb, isc)], -fac * sum);
+      }
+      for (int x = tid; x < NA * ND; x += CQ_THREADS) {  // K_ad
+         const int ia = (x / ND) / nsa, isa = (x / ND) % nsa, id = (x % ND) / nsd, isd = (x % ND) % nsd;
+         double sum = 0.0;
+         for (int ib = 0; ib < nlb; ib++)
+            for (int ic = 0; ic < nlc; ic++) {
+               const size_t o = (size_t)offs(0, ib, ic, 0, 2);
+               for (int isb = 0; isb < nsb; isb++)
+                  for (int isc = 0; isc < nsc; isc++)
+                     sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tbc, ldbc, isb, isc)];
+            }
+         unsafeAtomicAdd(&K[(size_t)offs(ia, 0, 0, id, 1) + blk_idx(Tad, ldad, isa, isd)], -fac * sum);
+      }
+      for (int x = tid; x < NA * NC; x += CQ_THREADS) {  // K_ac
+         const int ia = (x / NC) / nsa, isa = (x / NC) % nsa, ic = (x % NC) / nsc, isc = (x % NC) % nsc;
+         double sum = 0.0;
+         for (int ib = 0; ib < nlb; ib++)
+            for (int id = 0; id < nld; id++) {
+               const size_t o = (size_t)offs(0, ib, 0, id, 3);
+               for (int isb = 0; isb < nsb; isb++)
+                  for (int isd = 0; isd < nsd; isd++)
+                     sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tbd, ldbd, isb, isd)];
+            }
+         unsafeAtomicAdd(&K[(size_t)offs(ia, 0, ic, 0, 0) + blk_idx(Tac, ldac, isa, isc)], -fac * sum);
+      }
+   }
+}
+
+// one workgroup per set quartet, its classes in turn: the group's rows,
+// primitive lists and coefficients stay hot in the caches across classes
+__global__ __launch_bounds__(CQ_THREADS) void k_gqs(const ClassPlan *__restrict__ cps, const CqLayout *__restrict__ lays,
+                                                    const int *__restrict__ srcmaps, const TypePlan *__restrict__ tpp,
+                                                    DevTables t, BatchDev b, KTargets kt, double *gws, int gws_stride,
+                                                    int cls_bytes)
+{
+   const TypePlan &tp = *tpp;
+   const int tid = threadIdx.x;
+   // LDS: [class workspace, cls_bytes][class plan][coefficients nco_tot][primitive indices CQ_PCH]
+   ClassPlan *cpl = (ClassPlan *)((char *)lds_base() + cls_bytes);
+   double *co = (double *)((char *)cpl + CPL_BYTES);
+   uint32_t *ipb = (uint32_t *)(co + tp.nco_tot);
+   for (uint32_t gi = b.g0 + blockIdx.x; gi < b.g1; gi += gridDim.x) {
+      const KsGroup *ksrow = b.ksg + (size_t)gi * tp.nclass;
+      const GroupRec &g = b.groups[gi];
+      // the set quartet's contraction coefficients, shell by shell
+      for (int tc = 0; tc < 4; tc++) {
+         const int npt = ((g.np4 >> (4 * tc)) & 15) + 1;
+         for (int s = 0; s < tp.nsh[tc]; s++) {
+            const int n = tp.nl[tc][s] * MAXNP;
+            for (int i = tid; i < n; i += CQ_THREADS) {
+               const int inl = i / MAXNP, ip = i % MAXNP;
+               co[tp.co_off[tc][s] + i] = ip < npt ? b.data[g.Kof[tc][s] + inl * npt + ip] : 0.0;
             }
          }
       }
-      if (!reduce) continue;
-      __syncthreads();
-      // the same updates, summed over the set quartet's contractions first:
-      // K_bd += i P_ac, K_bc += i P_ad, K_ad += i P_bc, K_ac += i P_bd
-      const double fac = symm[(ks.flags >> 4) & 7] * kt.hf_fac;
-      if (fac == 0.0) continue;
-      const unsigned ldac = ks.ld01 & 0xffff, ldad = ks.ld01 >> 16, ldbc = ks.ld23 & 0xffff, ldbd = ks.ld23 >> 16;
-      const bool Tac = ks.flags & 1, Tad = ks.flags & 2, Tbc = ks.flags & 4, Tbd = ks.flags & 8;
-      const int NA = nla * nsa, NB = nlb * nsb, NC = nlc * nsc, ND = nld * nsd;
-      auto Iat = [&](int ia, int isa, int ib, int isb, int ic, int isc, int id, int isd) {
-         return Iall[(((ia * nlb + ib) * nlc + ic) * nld + id) * Ns + ((isa * nsb + isb) * nsc + isc) * nsd + isd];
-      };
-      auto offs = [&](int ia, int ib, int ic, int id, int q) {
-         switch (q) {
-            case 0: return (long long)ks.base[0] + (long long)ia * ks.step[0] + (long long)ic * ks.step[1];
-            case 1: return (long long)ks.base[1] + (long long)ia * ks.step[2] + (long long)id * ks.step[3];
-            case 2: return (long long)ks.base[2] + (long long)ib * ks.step[4] + (long long)ic * ks.step[5];
-            default: return (long long)ks.base[3] + (long long)ib * ks.step[6] + (long long)id * ks.step[7];
+      for (int k = 0; k < tp.nclass; k++) {
+         if ((ksrow[k].flags >> 16) == 0) continue;
+         const CqLayout &y = lays[k];
+         __syncthreads();
+         {
+            const int *src = (const int *)(cps + k);
+            int *dst = (int *)cpl;
+            for (int i = tid; i < (int)(sizeof(ClassPlan) / 4); i += CQ_THREADS) dst[i] = src[i];
          }
-      };
-      for (int s = 0; s < kt.nspin; s++) {
-         const double *P = kt.P[s];
-         double *K = kt.K[s];
-         for (int x = tid; x < NB * ND; x += Q_THREADS) {  // K_bd
-            const int ib = (x / ND) / nsb, isb = (x / ND) % nsb, id = (x % ND) / nsd, isd = (x % ND) % nsd;
-            double sum = 0.0;
-            for (int ia = 0; ia < nla; ia++)
-               for (int ic = 0; ic < nlc; ic++) {
-                  const size_t o = (size_t)offs(ia, 0, ic, 0, 0);
-                  for (int isa = 0; isa < nsa; isa++)
-                     for (int isc = 0; isc < nsc; isc++)
-                        sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tac, ldac, isa, isc)];
-               }
-            unsafeAtomicAdd(&K[(size_t)offs(0, ib, 0, id, 3) + blk_idx(Tbd, ldbd, isb, isd)], -fac * sum);
-         }
-         for (int x = tid; x < NB * NC; x += Q_THREADS) {  // K_bc
-            const int ib = (x / NC) / nsb, isb = (x / NC) % nsb, ic = (x % NC) / nsc, isc = (x % NC) % nsc;
-            double sum = 0.0;
-            for (int ia = 0; ia < nla; ia++)
-               for (int id = 0; id < nld; id++) {
-                  const size_t o = (size_t)offs(ia, 0, 0, id, 1);
-                  for (int isa = 0; isa < nsa; isa++)
-                     for (int isd = 0; isd < nsd; isd++)
-                        sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tad, ldad, isa, isd)];
-               }
-            unsafeAtomicAdd(&K[(size_t)offs(0, ib, ic, 0, 2) + blk_idx(Tbc, ldbc, isb, isc)], -fac * sum);
-         }
-         for (int x = tid; x < NA * ND; x += Q_THREADS) {  // K_ad
-            const int ia = (x / ND) / nsa, isa = (x / ND) % nsa, id = (x % ND) / nsd, isd = (x % ND) % nsd;
-            double sum = 0.0;
-            for (int ib = 0; ib < nlb; ib++)
-               for (int ic = 0; ic < nlc; ic++) {
-                  const size_t o = (size_t)offs(0, ib, ic, 0, 2);
-                  for (int isb = 0; isb < nsb; isb++)
-                     for (int isc = 0; isc < nsc; isc++)
-                        sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tbc, ldbc, isb, isc)];
-               }
-            unsafeAtomicAdd(&K[(size_t)offs(ia, 0, 0, id, 1) + blk_idx(Tad, ldad, isa, isd)], -fac * sum);
-         }
-         for (int x = tid; x < NA * NC; x += Q_THREADS) {  // K_ac
-            const int ia = (x / NC) / nsa, isa = (x / NC) % nsa, ic = (x % NC) / nsc, isc = (x % NC) % nsc;
-            double sum = 0.0;
-            for (int ib = 0; ib < nlb; ib++)
-               for (int id = 0; id < nld; id++) {
-                  const size_t o = (size_t)offs(0, ib, 0, id, 3);
-                  for (int isb = 0; isb < nsb; isb++)
-                     for (int isd = 0; isd < nsd; isd++)
-                        sum += Iat(ia, isa, ib, isb, ic, isc, id, isd) * P[o + blk_idx(Tbd, ldbd, isb, isd)];
-               }
-            unsafeAtomicAdd(&K[(size_t)offs(ia, 0, ic, 0, 0) + blk_idx(Tac, ldac, isa, isc)], -fac * sum);
-         }
+         __syncthreads();
+         double *WS = y.ws_global ? gws + (size_t)blockIdx.x * gws_stride : lds_base() + y.o_ws;
+         cq_one(*cpl, k, tp, t, b, kt, y, WS, srcmaps + y.o_src, gi, co, ipb);
       }
+      __syncthreads();
    }
 }
 
@@ -1071,47 +1119,78 @@ int launch_fixup(const TypePlan &tp, const DevTables &t, const BatchDev &b, uint
    if ((size_t)y.bytes > LDS_LIMIT) return -3;
    const unsigned grid = std::max(1u, std::min(n_fix, 4096u));
    hipLaunchKernelGGL(k_fm_r<true>, dim3(grid), dim3(64), y.bytes, (hipStream_t)stream, b.tp, t, b, y);
-   if (hipGetLastError() != hipSuccess) return -1;
-   hipLaunchKernelGGL(k_fix_eco, dim3(grid), dim3(256), 0, (hipStream_t)stream, b.tp, b);
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_eco(const TypePlan &tp, const BatchDev &b, void *stream)
+// LDS plan of one class in k_gqs: as many contractions per pass as fit in
+// `budget` doubles, the workspace in HBM when not even one does
+static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget)
 {
-   (void)tp;
-   if (b.r1 <= b.r0) return 0;
-   hipLaunchKernelGGL(k_eco, dim3(b.r1 - b.r0), dim3(ECO_THREADS), 0, (hipStream_t)stream, b.tp, b);
-   return hipGetLastError() == hipSuccess ? 0 : -1;
+   CqLayout y;
+   y.lws = 3 * cp.ycap + cp.Nc;
+   const int need_I = nq * cp.Ns;
+   y.reduce = need_I <= 2048 ? 1 : 0;
+   const int fixed = (y.reduce ? need_I : 0) + CQ_PCH;
+   y.ws_global = 0;
+   int lb = (budget - fixed) / (y.lws + CQ_PCH);
+   if (lb < 1) {
+      lb = 1;
+      y.ws_global = 1;
+   }
+   y.lb = std::min(lb, nq);
+   int off = 0;
+   y.o_ws = off;
+   if (!y.ws_global) off += y.lb * y.lws;
+   y.o_kp = off;
+   off += y.lb * CQ_PCH;
+   y.o_I = off;
+   if (y.reduce) off += need_I;
+   y.o_src = 0;
+   y.bytes = 8 * off;
+   return y;
 }
 
-static size_t hrr_lds_full(const ClassPlan &cp) { return sizeof(double) * (size_t)cp.hrr_ws; }
-
-size_t scratch_doubles(const ClassPlan &cp)
+void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::vector<CqLayout> &lays,
+                  std::vector<int> &srcmaps, int *lds_bytes, int *gws_stride)
 {
-   return hrr_lds_full(cp) > LDS_LIMIT ? (size_t)cp.hrr_ws * SCRATCH_GRID : 0;
+   // the budget of the type: what its largest class needs, within 64 KB
+   int budget = 512;
+   for (int k = 0; k < tp.nclass; k++) {
+      const int need = 3 * cps[k].ycap + cps[k].Nc + CQ_PCH + tp.cls_nl[k] * cps[k].Ns + CQ_PCH;
+      budget = std::max(budget, std::min(need, 8192 - 160 - tp.nco_tot - 4));
+   }
+   lays.clear();
+   srcmaps.clear();
+   *lds_bytes = 0;
+   *gws_stride = 0;
+   for (int k = 0; k < tp.nclass; k++) {
+      const ClassPlan &cp = cps[k];
+      CqLayout y = cq_layout(cp, tp.cls_nl[k], budget);
+      y.o_src = (int)srcmaps.size();
+      for (int e = cp.la; e <= cp.LA; e++)
+         for (int f = cp.lc; f <= cp.LC; f++)
+            for (int i = 0; i < nco(e) * nco(f); i++) srcmaps.push_back(tp.cpoff[e][f] + i);
+      *lds_bytes = std::max(*lds_bytes, y.bytes);
+      if (y.ws_global) *gws_stride = std::max(*gws_stride, y.lws);
+      lays.push_back(y);
+   }
 }
 
-int launch_hrr_sph_ks(const ClassPlan &cp, int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b,
-                      const KTargets &kt, Scratch s, void *stream)
+int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lays, const int *d_srcmaps, int lds_bytes,
+               int gws_stride, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s, void *stream)
 {
    if (b.g1 <= b.g0) return 0;
-   hipStream_t st = (hipStream_t)stream;
+   const int cls_bytes = (lds_bytes + 15) & ~15;
+   const size_t total = (size_t)cls_bytes + CPL_BYTES + 8 * (size_t)tp.nco_tot + 4 * CQ_PCH;
+   if (total > LDS_LIMIT) return -3;
    const uint32_t ng = b.g1 - b.g0;
-   const bool in_lds = hrr_lds_full(cp) <= LDS_LIMIT;
-   const size_t ws_bytes = in_lds ? hrr_lds_full(cp) : 0;
-   const int need = tp.cls_nl[cls_k] * cp.Ns;
-   int icap = (int)((LDS_LIMIT - ws_bytes) / sizeof(double));
-   icap = need <= icap ? need : 0;
-   const size_t lds = ws_bytes + sizeof(double) * (size_t)icap;
-   if (in_lds) {
-      const unsigned grid = std::min(ng, 1u << 20);
-      hipLaunchKernelGGL(k_hrr_sph_ks, dim3(grid), dim3(Q_THREADS), lds, st, cp, cls_k, b.tp, t, b, kt,
-                         (double *)nullptr, icap);
-   } else {
-      const unsigned grid = std::min(ng, SCRATCH_GRID);
-      if (!s.ptr || s.doubles < (size_t)cp.hrr_ws * grid) return -2;
-      hipLaunchKernelGGL(k_hrr_sph_ks, dim3(grid), dim3(Q_THREADS), lds, st, cp, cls_k, b.tp, t, b, kt, s.ptr, icap);
+   unsigned grid = std::min(ng, 1u << 20);
+   if (gws_stride) {
+      grid = std::min(grid, SCRATCH_GRID);
+      if (!s.ptr || s.doubles < (size_t)gws_stride * grid) return -2;
    }
+   hipLaunchKernelGGL(k_gqs, dim3(grid), dim3(CQ_THREADS), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
+                      b.tp, t, b, kt, s.ptr, gws_stride, cls_bytes);
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

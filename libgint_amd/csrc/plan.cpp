@@ -12,6 +12,7 @@
 // them, and the per-class bookkeeping (gates, contraction layout, checksum
 // multipliers).
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <map>
 #include <set>
@@ -227,6 +228,13 @@ void make_type_plan(TypePlan &tp)
             }
    p.nclass = k;
    p.ebytes = eoff;
+   int co = 0;
+   for (int t = 0; t < 4; t++)
+      for (int s2 = 0; s2 < p.nsh[t]; s2++) {
+         p.co_off[t][s2] = co;
+         co += p.nl[t][s2] * MAXNP;
+      }
+   p.nco_tot = co;
    p.nL = 0;
    for (int L : Lset) p.Ls[p.nL++] = L;
    for (k = 0; k < p.nclass; k++) {
@@ -253,6 +261,37 @@ void make_type_plan(TypePlan &tp)
          for (int ey = e - ex; ey >= 0; ey--) pf += 3LL * p.nkc * (ex + 1) * (ey + 1) * (e - ex - ey + 1);
    p.prim_flops = pf;
    tp = p;
+}
+
+void make_type_items(const TypePlan &tp, std::vector<uint32_t> &e1, std::vector<uint32_t> &e2)
+{
+   // cartesian components of shell L in order: lx descending, then ly descending
+   auto comps = [](int L) {
+      std::vector<std::array<int, 3>> v;
+      for (int lx = L; lx >= 0; lx--)
+         for (int ly = L - lx; ly >= 0; ly--) v.push_back({lx, ly, L - lx - ly});
+      return v;
+   };
+   std::vector<std::array<int, 3>> ket;
+   for (int f = tp.LC0; f <= tp.LC; f++)
+      for (auto &c : comps(f)) ket.push_back(c);
+   e1.clear();
+   for (int N = 0; N <= tp.LA; N++)
+      for (auto &h : comps(N))
+         for (auto &c : ket)
+            e1.push_back((uint32_t)h[0] | (uint32_t)h[1] << 4 | (uint32_t)h[2] << 8 | (uint32_t)c[0] << 12 |
+                         (uint32_t)c[1] << 16 | (uint32_t)c[2] << 20);
+   e2.assign(tp.ncp, 0u);
+   for (int e = tp.LA0; e <= tp.LA; e++) {
+      const auto ce = comps(e);
+      for (int f = tp.LC0; f <= tp.LC; f++)
+         for (int ie = 0; ie < nco(e); ie++)
+            for (int jl = 0; jl < nco(f); jl++) {
+               const int j = tp.cpoff[e][f] + ie * nco(f) + jl;
+               e2[j] = (uint32_t)ce[ie][0] | (uint32_t)ce[ie][1] << 4 | (uint32_t)ce[ie][2] << 8 |
+                       (uint32_t)(tp.kcoff[f] + jl) << 12;
+            }
+   }
 }
 
 }  // namespace gint
