@@ -121,6 +121,11 @@ void libgint_amd_plan_clear(void *handle);
 /* out[0] resident bytes, [1] quartet types, [2] chunks, [3] set quartets,
    [4] rows, [5] primitive quartets */
 void libgint_amd_plan_info(void *handle, uint64_t out[6]);
+/* work recorded and not yet on the device (host only, no GPU call):
+   out[0] quartet types, [1] set quartets, [2] rows, [3] primitive quartets,
+   [4] (set quartet, class) pairs with add_qrtt calls, [5] of them kept as
+   explicit lists because their offsets are not affine */
+void libgint_amd_queue_info(void *handle, uint64_t out[6]);
 /* evaluates the screening arithmetic on the GPU for a bitwise comparison
    with the host: out[4i..4i+3] = exp(x), sqrt|x|, 1/x, x*y/(x+y) */
 int libgint_amd_math_selftest(const double *x, const double *y, int n, double *out);

@@ -61,6 +61,7 @@ ABI = {
     "libgint_amd_plan_execute": (None, [_vp]),
     "libgint_amd_plan_clear": (None, [_vp]),
     "libgint_amd_plan_info": (None, [_vp, C.POINTER(C.c_uint64)]),
+    "libgint_amd_queue_info": (None, [_vp, C.POINTER(C.c_uint64)]),
     "libgint_amd_math_selftest": (_i, [_dp, _dp, _i, _dp]),
     "libgint_amd_math_host": (None, [_dp, _dp, _i, _dp]),
 }

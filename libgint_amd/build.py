@@ -99,7 +99,8 @@ def build_oracle(verbose=False):
 
 
 def build_ref(verbose=False):
-    """the reference's self-contained plain-C tables, compiled where they lie"""
+    """the reference's self-contained plain-C tables, compiled where they lie,
+    and its Fortran binding module linked with a caller against libgint.so"""
     if not os.path.isdir(REF_SRC):
         return None
     out = os.path.join(ROOT, "oracle", "_ref")
@@ -107,15 +108,23 @@ def build_ref(verbose=False):
     so = os.path.join(out, "libref.so")
     _run(["g++", "-O1", "-fPIC", "-shared", "-w", "-I" + REF_SRC, "-o", so,
           os.path.join(ROOT, "oracle", "ref_shim.cpp")], verbose)
+    fc = os.path.join(ROCM, "llvm", "bin", "amdflang")
+    mod = os.path.join(REF_SRC, "interface_libgint.F90")
+    if os.path.exists(fc) and os.path.exists(mod) and os.path.exists(LIB):
+        modir = os.path.join(BUILD, "fmod")
+        os.makedirs(modir, exist_ok=True)
+        _run([fc, "-O1", "-J", modir, "-o", os.path.join(out, "drive_libgint"), mod,
+              os.path.join(ROOT, "tests", "fortran", "drive_libgint.F90"), LIB,
+              "-Wl,-rpath,$ORIGIN/../../libgint_amd", "-Wl,-rpath," + os.path.join(ROCM, "lib")], verbose)
     return so
 
 
 def build_all(verbose=False):
     build_tcg(verbose)
     build_oracle(verbose)
-    build_ref(verbose)
     build_driver(verbose)
     build_lib(verbose)
+    build_ref(verbose)
 
 
 if __name__ == "__main__":

@@ -603,6 +603,23 @@ void Engine::plan_clear()
    plan_dev_.clear();
 }
 
+void Engine::queue_info(uint64_t out[6])
+{
+   for (int i = 0; i < 6; i++) out[i] = 0;
+   for (size_t ty = 0; ty < q_.size(); ty++) {
+      const TypeQueue &Q = q_[ty];
+      if (Q.empty()) continue;
+      out[0]++;
+      out[1] += Q.groups.size();
+      out[2] += Q.rows.size();
+      out[3] += Q.prm_ip.size();
+      for (const KsGroup &g : Q.ksg) {
+         if (ks_count(g.flags)) out[4]++;
+         if (g.flags & KS_EXPLICIT) out[5]++;
+      }
+   }
+}
+
 void Engine::plan_info(uint64_t out[6])
 {
    uint64_t bytes = 0, chunks = 0, groups = 0, rows = 0, prm = 0, nt = 0;

@@ -139,6 +139,7 @@ class Engine {
    void plan_execute();
    void plan_clear();
    void plan_info(uint64_t out[6]);
+   void queue_info(uint64_t out[6]);
    int np_of(int i) const { return i < (int)sets_.size() ? sets_[i].np : 0; }
    const double *exponents_of(int i) const { return &store_.data()[sets_[i].Z]; }
 

@@ -41,6 +41,8 @@ SIG = {
     "oracle_tcg": (C.c_int, [_vp, C.c_double, C.c_double, C.c_int, _dp]),
     "oracle_patch_grid": (None, [C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "oracle_boys_table": (None, [_dp]),
+    "oracle_patch_bw": (None, [_dp]),
+    "oracle_c2s": (None, [_dp]),
     "oracle_slot_key": (C.c_uint64, [C.c_int] * 15),
     "oracle_row_cart": (None, [_vp] + [C.c_int] * 11 + [C.POINTER(C.c_uint32), _dp]),
 }

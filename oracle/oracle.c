@@ -181,6 +181,26 @@ void oracle_patch_grid(int *grid_lo, int *grid_hi)
    }
 }
 
+/* bias and width of each patch, TG = (2X - B) W (the reference's
+   src/BW_by_patch.h holds the same numbers for the same boxes) */
+void oracle_patch_bw(double *bw)
+{
+   for (int p = 0; p < TCG_NPATCH; p++) {
+      const unsigned char *b = tcg_patch_box[p];
+      bw[4 * p + 0] = (double)(b[0] + b[1]) / 128.0;
+      bw[4 * p + 1] = 128.0 / (double)(b[1] - b[0]);
+      bw[4 * p + 2] = (double)(b[2] + b[3]) / 128.0;
+      bw[4 * p + 3] = 128.0 / (double)(b[3] - b[2]);
+   }
+}
+
+/* the cartesian -> spherical coefficients the oracle applies (l = 0..3) */
+void oracle_c2s(double *out)
+{
+   static const double v[GINT_C2S_SIZE] = {GINT_C2S_VALUES};
+   for (int i = 0; i < GINT_C2S_SIZE; i++) out[i] = v[i];
+}
+
 /* ------------------------------------------------------------------ */
 /* engine state                                                        */
 /* ------------------------------------------------------------------ */
@@ -247,13 +267,7 @@ void *oracle_create_handle(void)
    o->hf_fac = 1.0;
    for (int i = 0; i < 9; i++) o->cell[i] = o->cell[9 + i] = (i % 4 == 0) ? 1.0 : 0.0;
    oracle_patch_grid(o->grid_lo, o->grid_hi);
-   for (int p = 0; p < TCG_NPATCH; p++) {
-      const unsigned char *b = tcg_patch_box[p];
-      o->bw[4 * p + 0] = (double)(b[0] + b[1]) / 128.0;
-      o->bw[4 * p + 1] = 128.0 / (double)(b[1] - b[0]);
-      o->bw[4 * p + 2] = (double)(b[2] + b[3]) / 128.0;
-      o->bw[4 * p + 3] = 128.0 / (double)(b[3] - b[2]);
-   }
+   oracle_patch_bw(o->bw);
    return o;
 }
 

@@ -64,6 +64,8 @@ void oracle_boys(int nmax, double T, double *f, double fac);
    returns the branch taken (0 zero, 1 Boys, 2 patch) */
 int oracle_tcg(void *h, double R, double T, int nder, double *G);
 void oracle_patch_grid(int *grid_lo, int *grid_hi);
+void oracle_patch_bw(double *bw /* 207*4 */);
+void oracle_c2s(double *out /* 110 */);
 void oracle_boys_table(double *ftable /* 121*28 */);
 
 /* contracted cartesian integrals [nlabcd][Nc] (layout [a][b][c][d]) of
