@@ -100,13 +100,14 @@ void libgint_amd_enable_stats(void *handle, int on);
 void libgint_amd_stats(void *handle, uint64_t out[8]);
 /* per-kernel device time with HIP events on the handle's stream */
 void libgint_amd_enable_timing(void *handle, int on);
-/* out[0..3] ms in fm_r / (unused) / the quartet kernel gqs / fix-up,
-   out[4..7] launches, out[10] algorithmic FLOPs of gqs (out[8], out[9],
-   out[11] = 0: the fm_r FLOPs are libgint_amd_stats out[6]) */
+/* out[0..3] ms in fm_r / the thread-per-contraction quartet kernels tq /
+   the per-set-quartet quartet kernel gqs / fix-up, out[4..7] launches,
+   out[9], out[10] algorithmic FLOPs of tq and gqs (out[8] and out[11] = 0:
+   the fm_r FLOPs are libgint_amd_stats out[6]) */
 void libgint_amd_timing(void *handle, double out[12]);
 void libgint_amd_reset_counters(void *handle);
-/* ms per launch key of one category of libgint_amd_timing: fm_r (0) and the
-   quartet kernel (2) by the quartet type's total angular momentum */
+/* ms per launch key of one category of libgint_amd_timing (0 fm_r, 1 tq,
+   2 gqs) by the quartet type's total angular momentum */
 void libgint_amd_timing_detail(void *handle, int cat, double out[256]);
 /* bit 0: route every surviving slot through the class-dependent screening
    path (a test of that path; results are unchanged) */
@@ -126,6 +127,20 @@ void libgint_amd_plan_info(void *handle, uint64_t out[6]);
    [4] (set quartet, class) pairs with add_qrtt calls, [5] of them kept as
    explicit lists because their offsets are not affine */
 void libgint_amd_queue_info(void *handle, uint64_t out[6]);
+/* reads CP2K's truncated-Coulomb coefficient file (the T_C_G_DATA
+   t_c_g.dat) as reference src/t_c_g0_n.cpp:29-51 (read_c0) does: 207
+   patches of 22 x 105 white-space separated numbers, patch after patch.
+   Keeps the first (nder+1)*105 numbers of each patch in C0, row stride
+   *ld_C0 = (nder+1)*105, ready for libgint_set_Potential_Truncated (host
+   only, no GPU call).  Returns the number of doubles stored, 207 * *ld_C0,
+   or -1 the file cannot be opened, -2 nder outside 0..21, -3 capacity below
+   207*(nder+1)*105, -4 the file ends early or holds a token that is not a
+   number (the reference aborts on these) */
+int libgint_amd_read_c0(const char *path, int nder, double *C0, int capacity, int *ld_C0);
+/* writes a full table (ld_C0 = 2310, orders 0..21) in that format, one
+   patch per line, 17 significant digits so that reading it back is exact;
+   returns 0, -1 on an I/O error, -2 on a bad argument */
+int libgint_amd_write_c0(const char *path, const double *C0, int ld_C0);
 /* evaluates the screening arithmetic on the GPU for a bitwise comparison
    with the host: out[4i..4i+3] = exp(x), sqrt|x|, 1/x, x*y/(x+y) */
 int libgint_amd_math_selftest(const double *x, const double *y, int n, double *out);

@@ -64,6 +64,8 @@ ABI = {
     "libgint_amd_queue_info": (None, [_vp, C.POINTER(C.c_uint64)]),
     "libgint_amd_math_selftest": (_i, [_dp, _dp, _i, _dp]),
     "libgint_amd_math_host": (None, [_dp, _dp, _i, _dp]),
+    "libgint_amd_read_c0": (_i, [C.c_char_p, _i, _dp, _i, C.POINTER(_i)]),
+    "libgint_amd_write_c0": (_i, [C.c_char_p, _dp, _i]),
 }
 
 _lib = None
@@ -82,6 +84,32 @@ def load():
             f.argtypes = args
         _lib = lib
     return _lib
+
+
+def read_c0(path, nder=21):
+    """CP2K's t_c_g.dat coefficient file -> (C0 as 207 x (nder+1)*105 float64,
+    ld_C0), through the library's reader; raises on a malformed file"""
+    import numpy as np
+
+    ld = (nder + 1) * 105
+    c0 = np.empty((207, ld))
+    got = C.c_int(0)
+    n = load().libgint_amd_read_c0(os.fsencode(path), nder, c0.ctypes.data_as(_dp), c0.size, C.byref(got))
+    if n < 0:
+        raise ValueError("libgint_amd_read_c0(%s, nder=%d) failed with %d" % (path, nder, n))
+    return c0, got.value
+
+
+def write_c0(path, c0):
+    """writes a full 207 x 2310 table in the t_c_g.dat format"""
+    import numpy as np
+
+    c0 = np.ascontiguousarray(c0, dtype=np.float64)
+    if c0.shape != (207, 2310):
+        raise ValueError("write_c0 takes the full 207 x 2310 table")
+    rc = load().libgint_amd_write_c0(os.fsencode(path), c0.ctypes.data_as(_dp), 2310)
+    if rc != 0:
+        raise OSError("libgint_amd_write_c0(%s) failed with %d" % (path, rc))
 
 
 def reference_symbols():

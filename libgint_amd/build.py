@@ -22,8 +22,8 @@ TCG_BIN = os.path.join(HERE, "data", "tcg_c0.bin")
 TCG_SHA256 = None
 REF_SRC = "/root/reference/src"
 
-HOST_SRC = ["engine.cpp", "abi.cpp", "plan.cpp", "boys_table.cpp"]
-DEV_SRC = ["kernels.hip"]
+HOST_SRC = ["engine.cpp", "abi.cpp", "plan.cpp", "boys_table.cpp", "c0_file.cpp"]
+DEV_SRC = ["kernels.hip", "kernels_tq.hip"]
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 HOSTFLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",

@@ -198,12 +198,16 @@ struct CqLayout {
    int o_src;       // first entry of the class's block map (class element -> type element) in the type's maps
    int reduce;
    int ws_global;   // workspace in HBM (gws + blockIdx * lb * lws)
+   int skip;        // the class runs in its thread-per-contraction kernel instead
    int bytes;
 };
 
 // launchers (kernels.hip), asynchronous on `stream`; 0 or a negative code
 int launch_fm_r(const TypePlan &tp, const DevTables &t, const BatchDev &b, void *stream);
 int launch_fixup(const TypePlan &tp, const DevTables &t, const BatchDev &b, uint32_t n_fix, void *stream);
+// thread-per-contraction kernels of the small classes (kernels_tq.hip)
+bool tq_has(int code);
+int launch_tq(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt, void *stream);
 // per-class LDS plans and block maps of a type for the quartet kernel
 void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::vector<CqLayout> &lays,
                   std::vector<int> &srcmaps, int *lds_bytes, int *gws_stride);

@@ -775,8 +775,9 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
          for (int k = 0; k < tp.nclass; k++) {
             const double nq = ks_count(Q.ksg[gi * tp.nclass + k].flags);
             const ClassPlan &cp = ti.cplans[k];
-            c.k3_flops += nq * (2.0 * (double)gp * cp.ncp + (double)(g.row1 - g.row0) * cp.hrr_flops + cp.sph_flops +
-                                9.0 * cp.Ns);
+            const double f = nq * (2.0 * (double)gp * cp.ncp + (double)(g.row1 - g.row0) * cp.hrr_flops + cp.sph_flops +
+                                   9.0 * cp.Ns);
+            (ti.lays[k].skip ? c.tq_flops : c.k3_flops) += f;
          }
       }
       if (c.g1 > c.g0) D.chunks.push_back(c);
@@ -915,12 +916,25 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
    rc = launch_fixup(tp, t, b, (debug_ & 1) ? (uint32_t)fix_cap_ : 64u, st);
    if (rc) fatal("fix-up launch", rc);
    ev_end(3);
-   ev_begin(2);
+   ev_begin(1);
    ev_key_ = tp.L;
-   rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, t, b, kt, Scratch{d_scratch_, scratch_cap_},
-                   st);
-   if (rc) fatal("gqs launch", rc);
-   ev_end(2);
+   for (int k = 0; k < tp.nclass; k++) {
+      if (!td.class_used[k] || !ti.lays[k].skip) continue;
+      rc = launch_tq(k, tp, t, b, kt, st);
+      if (rc) fatal("tq launch", rc);
+   }
+   ev_end(1);
+   bool any_gqs = false;
+   for (int k = 0; k < tp.nclass; k++) any_gqs = any_gqs || (td.class_used[k] && !ti.lays[k].skip);
+   if (any_gqs) {
+      ev_begin(2);
+      ev_key_ = tp.L;
+      rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, t, b, kt, Scratch{d_scratch_, scratch_cap_},
+                      st);
+      if (rc) fatal("gqs launch", rc);
+      ev_end(2);
+   }
+   timing_.flops[1] += c.tq_flops;
    timing_.flops[2] += c.k3_flops;
    n_prmc_ += (uint64_t)(c.p1 - c.p0) * tp.nclass;
    n_slots_ += (uint64_t)(c.p1 - c.p0) * (uint64_t)t.Ng * tp.nclass;

@@ -59,7 +59,7 @@ struct TypeQueue {
 
 struct Chunk {
    uint32_t g0, g1, r0, r1, p0, p1;
-   double k3_flops;
+   double tq_flops, k3_flops;
 };
 
 // one type's work on the device
@@ -93,9 +93,9 @@ struct TypeInfo {
 };
 
 struct Timing {
-   double ms[4] = {0, 0, 0, 0};        // fm_r, (unused), cqs, fixup
+   double ms[4] = {0, 0, 0, 0};        // fm_r, tq, gqs, fixup
    uint64_t launches[4] = {0, 0, 0, 0};
-   double flops[4] = {0, 0, 0, 0};     // host-counted algorithmic FLOPs of cqs in [2]
+   double flops[4] = {0, 0, 0, 0};     // host-counted algorithmic FLOPs of tq [1] and gqs [2]
 };
 
 class Engine {

@@ -314,6 +314,7 @@ struct FmLayout {
    int o_sv;    // per lane: X Y Z of P-Q-g, TG1 (or T), TG2
    int o_R;     // Hermite layers, 2 x SB x nh
    int o_int;   // byte offset of 64 lane infos (branch | patch << 2) and 64 survivor lanes
+   int o_desc;  // byte offset of the transform item descriptors (e1 then e2), 0 if they stay in HBM
    int bytes;
 };
 
@@ -335,6 +336,13 @@ static FmLayout fm_layout(const TypePlan &tp, bool fix)
    const int dbl = std::max(cell_phase, e_phase);
    y.o_int = 8 * dbl;
    y.bytes = y.o_int + 4 * 128;
+   // the transform's item descriptors, once per workgroup, when small
+   const int ndesc = tp.nbh * tp.nkc + tp.ncp;
+   y.o_desc = 0;
+   if (!fix && 4 * ndesc <= 12288) {
+      y.o_desc = y.bytes;
+      y.bytes += 4 * ndesc;
+   }
    return y;
 }
 
@@ -353,6 +361,16 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
    int *svi = (int *)((char *)lds + lay.o_int);
    int *svl = svi + 64;
    const uint32_t nwork = FIX ? min(b.fix_count[0], b.fix_cap) : (b.p1 - b.p0);
+   const uint32_t *e1 = b.e1, *e2 = b.e2;
+   if (lay.o_desc) {
+      uint32_t *d = (uint32_t *)((char *)lds + lay.o_desc);
+      const int n1 = tp.nbh * tp.nkc;
+      for (int i = threadIdx.x; i < n1; i += 64) d[i] = b.e1[i];
+      for (int i = threadIdx.x; i < tp.ncp; i += 64) d[n1 + i] = b.e2[i];
+      e1 = d;
+      e2 = d + n1;
+      __syncthreads();
+   }
 
    for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
       uint32_t p;
@@ -632,7 +650,7 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
          const int nkc = tp.nkc;
          // W[tuv][f] = sum E^f_tau E^f_nu E^f_phi R[t+tau, u+nu, v+phi]
          for (int idx = lane; idx < tp.nbh * nkc; idx += 64) {
-            const uint32_t it = b.e1[idx];
+            const uint32_t it = e1[idx];
             const int tt = it & 15, uu = (it >> 4) & 15, vv = (it >> 8) & 15;
             const int fx = (it >> 12) & 15, fy = (it >> 16) & 15, fz = (it >> 20) & 15;
             const double *Fx = Fk + fx * (fx + 1) / 2, *Fy = Fk + 28 + fy * (fy + 1) / 2, *Fz = Fk + 56 + fz * (fz + 1) / 2;
@@ -652,7 +670,7 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
          __syncthreads();
          // (e0|f0) = sum E^e_t E^e_u E^e_v W[tuv][f]
          for (int j = lane; j < tp.ncp; j += 64) {
-            const uint32_t it = b.e2[j];
+            const uint32_t it = e2[j];
             const int ex = it & 15, ey = (it >> 4) & 15, ez = (it >> 8) & 15, col = (int)(it >> 12);
             const double *Ea = Ex + ex * (ex + 1) / 2, *Eb = Ex + 28 + ey * (ey + 1) / 2, *Ec = Ex + 56 + ez * (ez + 1) / 2;
             double s = 0.0;
@@ -1077,7 +1095,7 @@ __global__ __launch_bounds__(CQ_THREADS) void k_gqs(const ClassPlan *__restrict_
          }
       }
       for (int k = 0; k < tp.nclass; k++) {
-         if ((ksrow[k].flags >> 16) == 0) continue;
+         if ((ksrow[k].flags >> 16) == 0 || lays[k].skip) continue;
          const CqLayout &y = lays[k];
          __syncthreads();
          {
@@ -1109,7 +1127,10 @@ int launch_fm_r(const TypePlan &tp, const DevTables &t, const BatchDev &b, void 
    if (b.p1 <= b.p0) return 0;
    const FmLayout y = fm_layout(tp, false);
    if ((size_t)y.bytes > LDS_LIMIT) return -3;
-   hipLaunchKernelGGL(k_fm_r<false>, dim3(b.p1 - b.p0), dim3(64), y.bytes, (hipStream_t)stream, b.tp, t, b, y);
+   // workgroups stay resident and stride over the primitive quartets, so the
+   // descriptors staged in LDS serve many of them
+   const unsigned grid = std::min(b.p1 - b.p0, 16384u);
+   hipLaunchKernelGGL(k_fm_r<false>, dim3(grid), dim3(64), y.bytes, (hipStream_t)stream, b.tp, t, b, y);
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1156,6 +1177,7 @@ void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::ve
    // the budget of the type: what its largest class needs, within 64 KB
    int budget = 512;
    for (int k = 0; k < tp.nclass; k++) {
+      if (tq_has(tp.cls_code[k])) continue;
       const int need = 3 * cps[k].ycap + cps[k].Nc + CQ_PCH + tp.cls_nl[k] * cps[k].Ns + CQ_PCH;
       budget = std::max(budget, std::min(need, 8192 - 160 - tp.nco_tot - 4));
    }
@@ -1166,12 +1188,15 @@ void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::ve
    for (int k = 0; k < tp.nclass; k++) {
       const ClassPlan &cp = cps[k];
       CqLayout y = cq_layout(cp, tp.cls_nl[k], budget);
+      y.skip = tq_has(tp.cls_code[k]) ? 1 : 0;
       y.o_src = (int)srcmaps.size();
       for (int e = cp.la; e <= cp.LA; e++)
          for (int f = cp.lc; f <= cp.LC; f++)
             for (int i = 0; i < nco(e) * nco(f); i++) srcmaps.push_back(tp.cpoff[e][f] + i);
-      *lds_bytes = std::max(*lds_bytes, y.bytes);
-      if (y.ws_global) *gws_stride = std::max(*gws_stride, y.lws);
+      if (!y.skip) {
+         *lds_bytes = std::max(*lds_bytes, y.bytes);
+         if (y.ws_global) *gws_stride = std::max(*gws_stride, y.lws);
+      }
       lays.push_back(y);
    }
 }

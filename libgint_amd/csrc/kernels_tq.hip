@@ -1,0 +1,395 @@
+// Thread-per-contraction quartet kernel for the small angular-momentum
+// classes, one instantiation per class (la lb|lc ld).
+//
+// k_tq<la,lb,lc,ld>: one lane per (set quartet, contraction) of the class.
+// The lane contracts each (n1,n2) row's primitive blocks (reference
+// src/compute_ECO.cpp:12-214), runs the HRR of the row (d -> c, then
+// b -> a, src/compute_HRR.cpp) as straight-line code on registers, sums
+// the rows, transforms to spherical (src/compute_SPH.cpp:58-237) and makes
+// the four K updates per integral (src/compute_KS.cpp:51-115).  Every index
+// and every c2s coefficient is a compile-time constant, so zero coefficients
+// cost nothing and there are no barriers: the lanes of a wavefront that
+// share a set quartet read the same primitive blocks (broadcast loads).
+#include <hip/hip_runtime.h>
+
+#include "device.hpp"
+#include "gint_math.h"
+#include "c2s_table.h"
+
+namespace gint {
+namespace tq {
+
+__host__ __device__ constexpr int nco(int l) { return (l + 1) * (l + 2) / 2; }
+
+// component d (x y z) of cartesian function i of shell L (descending x, then y)
+__host__ __device__ constexpr int comp(int L, int i, int d)
+{
+   int n = 0;
+   for (int lx = L; lx >= 0; lx--)
+      for (int ly = L - lx; ly >= 0; ly--) {
+         if (n == i) return d == 0 ? lx : (d == 1 ? ly : L - lx - ly);
+         n++;
+      }
+   return 0;
+}
+__host__ __device__ constexpr int cidx(int L, int lx, int lz) { return (L - lx) * (L - lx + 1) / 2 + lz; }
+__host__ __device__ constexpr int first_dir(int L, int i) { return comp(L, i, 0) > 0 ? 0 : (comp(L, i, 1) > 0 ? 1 : 2); }
+__host__ __device__ constexpr int shift_idx(int L, int i, int d, int by)
+{
+   const int lx = comp(L, i, 0) + (d == 0 ? by : 0);
+   const int lz = comp(L, i, 2) + (d == 2 ? by : 0);
+   return cidx(L + by, lx, lz);
+}
+__host__ __device__ constexpr double c2s(int l, int m, int k)
+{
+   constexpr double v[] = {GINT_C2S_VALUES};
+   constexpr int off[4] = {GINT_C2S_OFF_0, GINT_C2S_OFF_1, GINT_C2S_OFF_2, GINT_C2S_OFF_3};
+   return v[off[l] + m * nco(l) + k];
+}
+
+// layouts of one class, as in ClassPlan (plan.cpp), at compile time
+template <int la, int lb, int lc, int ld>
+struct Cls {
+   static constexpr int LA = la + lb, LC = lc + ld;
+   static constexpr int Nc = nco(la) * nco(lb) * nco(lc) * nco(ld);
+   static constexpr int Ns = (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1) * (2 * ld + 1);
+   static constexpr int ncd = nco(lc) * nco(ld);
+   // ket layer dd: blocks (e, c), e in [la, LA], c in [lc, LC - dd], e outer
+   static constexpr int koff(int dd, int e, int c)
+   {
+      int off = 0;
+      for (int e2 = la; e2 <= LA; e2++)
+         for (int c2 = lc; c2 <= LC - dd; c2++) {
+            if (e2 == e && c2 == c) return off;
+            off += nco(e2) * nco(c2) * nco(dd);
+         }
+      return off;
+   }
+   static constexpr int ksize(int dd) { return koff(dd, LA + 1, lc); }
+   // bra layer bb: blocks a in [la, LA - bb]; layer 0 is the last ket layer
+   static constexpr int boff(int bb, int a)
+   {
+      int off = 0;
+      for (int a2 = la; a2 <= LA - bb; a2++) {
+         if (a2 == a) return off;
+         off += nco(a2) * nco(bb) * ncd;
+      }
+      return off;
+   }
+   static constexpr int bsize(int bb) { return boff(bb, LA - bb + 1); }
+   static constexpr int ncp = koff(0, LA + 1, lc);
+   static constexpr int cap()
+   {
+      int c = ncp > Nc ? ncp : Nc;
+      for (int dd = 1; dd <= ld; dd++) c = ksize(dd) > c ? ksize(dd) : c;
+      for (int bb = 1; bb <= lb; bb++) c = bsize(bb) > c ? bsize(bb) : c;
+      return c;
+   }
+};
+
+// ket HRR step: layer dd (out) from layer dd-1 (in)
+template <int la, int lb, int lc, int ld, int dd>
+__device__ __forceinline__ void ket_step(const double *in, double *out, const double CD[3])
+{
+   using C = Cls<la, lb, lc, ld>;
+#pragma unroll
+   for (int e = la; e <= C::LA; e++)
+#pragma unroll
+      for (int c = lc; c <= C::LC - dd; c++)
+#pragma unroll
+         for (int ie = 0; ie < nco(e); ie++)
+#pragma unroll
+            for (int ic = 0; ic < nco(c); ic++)
+#pragma unroll
+               for (int id = 0; id < nco(dd); id++) {
+                  const int dir = first_dir(dd, id);
+                  const int idm = shift_idx(dd, id, dir, -1);
+                  const int icp = shift_idx(c, ic, dir, 1);
+                  const double v1 = in[C::koff(dd - 1, e, c + 1) + (ie * nco(c + 1) + icp) * nco(dd - 1) + idm];
+                  const double v0 = in[C::koff(dd - 1, e, c) + (ie * nco(c) + ic) * nco(dd - 1) + idm];
+                  out[C::koff(dd, e, c) + (ie * nco(c) + ic) * nco(dd) + id] = v1 + CD[dir] * v0;
+               }
+}
+
+// bra HRR step: layer bb (out) from layer bb-1 (in)
+template <int la, int lb, int lc, int ld, int bb>
+__device__ __forceinline__ void bra_step(const double *in, double *out, const double AB[3])
+{
+   using C = Cls<la, lb, lc, ld>;
+#pragma unroll
+   for (int a = la; a <= C::LA - bb; a++)
+#pragma unroll
+      for (int ia = 0; ia < nco(a); ia++)
+#pragma unroll
+         for (int ib = 0; ib < nco(bb); ib++)
+#pragma unroll
+            for (int icd = 0; icd < C::ncd; icd++) {
+               const int dir = first_dir(bb, ib);
+               const int ibm = shift_idx(bb, ib, dir, -1);
+               const int iap = shift_idx(a, ia, dir, 1);
+               const double v1 = in[C::boff(bb - 1, a + 1) + (iap * nco(bb - 1) + ibm) * C::ncd + icd];
+               const double v0 = in[C::boff(bb - 1, a) + (ia * nco(bb - 1) + ibm) * C::ncd + icd];
+               out[C::boff(bb, a) + (ia * nco(bb) + ib) * C::ncd + icd] = v1 + AB[dir] * v0;
+            }
+}
+
+// HRR of one row: E (layer 0) -> (la lb|lc ld) cartesian added to acc
+template <int la, int lb, int lc, int ld>
+__device__ __forceinline__ void hrr_add(double *E, double *T, double *acc, const double AB[3], const double CD[3])
+{
+   using C = Cls<la, lb, lc, ld>;
+   double *bufs[2] = {E, T};
+   int cur = 0;
+   if constexpr (ld >= 1) { ket_step<la, lb, lc, ld, 1>(bufs[cur], bufs[cur ^ 1], CD); cur ^= 1; }
+   if constexpr (ld >= 2) { ket_step<la, lb, lc, ld, 2>(bufs[cur], bufs[cur ^ 1], CD); cur ^= 1; }
+   if constexpr (ld >= 3) { ket_step<la, lb, lc, ld, 3>(bufs[cur], bufs[cur ^ 1], CD); cur ^= 1; }
+   if constexpr (lb >= 1) { bra_step<la, lb, lc, ld, 1>(bufs[cur], bufs[cur ^ 1], AB); cur ^= 1; }
+   if constexpr (lb >= 2) { bra_step<la, lb, lc, ld, 2>(bufs[cur], bufs[cur ^ 1], AB); cur ^= 1; }
+   if constexpr (lb >= 3) { bra_step<la, lb, lc, ld, 3>(bufs[cur], bufs[cur ^ 1], AB); cur ^= 1; }
+   const double *res = bufs[cur];
+#pragma unroll
+   for (int i = 0; i < C::Nc; i++) acc[i] += res[i];
+}
+
+// one stage of cart -> sph: in [k][n] -> out [n][m] for shell l
+template <int l, int n>
+__device__ __forceinline__ void sph_stage(const double *in, double *out)
+{
+#pragma unroll
+   for (int j = 0; j < n; j++)
+#pragma unroll
+      for (int i = 0; i < 2 * l + 1; i++) {
+         double v = 0.0;
+#pragma unroll
+         for (int q = 0; q < nco(l); q++)
+            if (c2s(l, i, q) != 0.0) v += c2s(l, i, q) * in[q * n + j];
+         out[j * (2 * l + 1) + i] = v;
+      }
+}
+
+struct TqArgs {
+   int nq_all;          // contractions of the class
+   int nclass;
+   int bsrc[4][4];      // type-block offset of class block (la+e, lc+f)
+   int nla, nlb, nlc, nld;
+   int sa, sb, sc, sd;  // shells of the class on the four centres
+};
+
+template <int la, int lb, int lc, int ld>
+__global__ __launch_bounds__(64) void k_tq(int cls_k, DevTables t, BatchDev b, KTargets kt, TqArgs y)
+{
+   using C = Cls<la, lb, lc, ld>;
+   constexpr int CAP = C::cap();
+   const uint32_t it = blockIdx.x * 64u + threadIdx.x;
+   const uint32_t gi = b.g0 + it / (uint32_t)y.nq_all;
+   const int l = (int)(it % (uint32_t)y.nq_all);
+   if (gi >= b.g1) return;
+   const KsGroup ks = b.ksg[(size_t)gi * y.nclass + cls_k];
+   const int nq = (int)(ks.flags >> 16);
+   if (l >= nq) return;
+   const GroupRec &g = b.groups[gi];
+   const int ia = l / (y.nlb * y.nlc * y.nld), ib = (l / (y.nlc * y.nld)) % y.nlb;
+   const int ic = (l / y.nld) % y.nlc, id = l % y.nld;
+   const int npa = (g.np4 & 15) + 1, npb = ((g.np4 >> 4) & 15) + 1, npc = ((g.np4 >> 8) & 15) + 1,
+             npd = ((g.np4 >> 12) & 15) + 1;
+   const double *Ka = b.data + g.Kof[0][y.sa] + ia * npa, *Kb = b.data + g.Kof[1][y.sb] + ib * npb;
+   const double *Kc = b.data + g.Kof[2][y.sc] + ic * npc, *Kd = b.data + g.Kof[3][y.sd] + id * npd;
+   const int ncpt = b.tp->ncp;
+   const uint32_t nfix = min(b.fix_count[0], b.fix_cap);
+   const int gate = b.tp->cls_gate[cls_k];
+
+   double acc[C::Nc];
+#pragma unroll
+   for (int i = 0; i < C::Nc; i++) acc[i] = 0.0;
+   for (uint32_t r = g.row0; r < g.row1; r++) {
+      const RowRec row = b.rows[r];
+      double E[CAP], T[CAP];
+#pragma unroll
+      for (int i = 0; i < C::ncp; i++) E[i] = 0.0;
+      const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * ncpt;
+      const uint32_t *IP = b.prm_ip + row.prm_off;
+      // eight primitive quartets at a time: their index loads, then their
+      // coefficient loads, then their blocks, each group issued together
+      constexpr int PB = 8;
+      for (uint32_t p0 = 0; p0 < row.n_prm; p0 += PB) {
+         uint32_t ipv[PB];
+         double kp[PB];
+#pragma unroll
+         for (int q = 0; q < PB; q++) ipv[q] = p0 + q < row.n_prm ? IP[p0 + q] : 0u;
+#pragma unroll
+         for (int q = 0; q < PB; q++) {
+            const uint32_t ip = ipv[q];
+            kp[q] = p0 + q < row.n_prm
+                       ? Ka[ip & 15] * Kb[(ip >> 4) & 15] * Kc[(ip >> 8) & 15] * Kd[(ip >> 12) & 15]
+                       : 0.0;
+         }
+#pragma unroll
+         for (int q = 0; q < PB; q++) {
+            if (p0 + q >= row.n_prm) break;
+            const double *A = A0 + (size_t)(p0 + q) * ncpt;
+#pragma unroll
+            for (int e = la; e <= C::LA; e++)
+#pragma unroll
+               for (int f = lc; f <= C::LC; f++) {
+                  const double *src = A + y.bsrc[e - la][f - lc];
+#pragma unroll
+                  for (int i = 0; i < nco(e) * nco(f); i++) E[C::koff(0, e, f) + i] += kp[q] * src[i];
+               }
+         }
+      }
+      // slots of this row whose screening depends on the class (rare)
+      for (uint32_t e2 = 0; e2 < nfix; e2++) {
+         const uint32_t p = b.fix[2 * e2];
+         if (b.prm_row[p] != r) continue;
+         const uint32_t ip = b.prm_ip[p];
+         const double kp = Ka[ip & 15] * Kb[(ip >> 4) & 15] * Kc[(ip >> 8) & 15] * Kd[(ip >> 12) & 15];
+         const double *src = b.fix_ws + (size_t)e2 * (ncpt + 16);
+         if (((b.fix[2 * e2 + 1] >> 8) >> gate) & 1u) {
+#pragma unroll
+            for (int e = la; e <= C::LA; e++)
+#pragma unroll
+               for (int f = lc; f <= C::LC; f++)
+#pragma unroll
+                  for (int i = 0; i < nco(e) * nco(f); i++)
+                     E[C::koff(0, e, f) + i] += kp * src[y.bsrc[e - la][f - lc] + i];
+         } else if (la == 0 && lc == 0) {
+            // a screened slot keeps its Fm[0] (reference src/compute_VRR.cpp:384-385)
+            E[0] += kp * src[ncpt + gate];
+         }
+      }
+      double AB[3], CD[3];
+      {
+         const double *Ao = b.data + g.R[0], *Bo = b.data + g.R[1], *Co = b.data + g.R[2], *Do = b.data + g.R[3];
+         const int n1 = row.n12 & 0xffff, n2 = row.n12 >> 16;
+         double ABs[3], CDs[3];
+         gint_pbc(Ao, Bo, t.cell, ABs);
+         gint_pbc(Co, Do, t.cell, CDs);
+#pragma unroll
+         for (int k = 0; k < 3; k++) {
+            const double B = Ao[k] + ABs[k] + t.neighs[n1 * 3 + k];
+            const double D = Co[k] + CDs[k] + t.neighs[n2 * 3 + k];
+            AB[k] = Ao[k] - B;
+            CD[k] = Co[k] - D;
+         }
+      }
+      hrr_add<la, lb, lc, ld>(E, T, acc, AB, CD);
+   }
+   // cartesian -> spherical: four stages, each moves the leading index last
+   double s0[C::Nc], s1[C::Nc];
+   constexpr int n0 = nco(lb) * nco(lc) * nco(ld);
+   constexpr int n1 = nco(lc) * nco(ld) * (2 * la + 1);
+   constexpr int n2 = nco(ld) * (2 * la + 1) * (2 * lb + 1);
+   constexpr int n3 = (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1);
+   sph_stage<la, n0>(acc, s0);
+   sph_stage<lb, n1>(s0, s1);
+   sph_stage<lc, n2>(s1, s0);
+   sph_stage<ld, n3>(s0, s1);
+   // exchange update, 4 atomics per spherical integral
+   size_t off[4];
+   uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
+   if (ks.flags & KS_EXPLICIT) {
+      const KsRec x = b.ksx[ks.expl + l];
+      for (int q = 0; q < 4; q++) off[q] = x.off[q];
+      ld01 = x.ld01;
+      ld23 = x.ld23;
+      flags = x.flags;
+   } else {
+      off[0] = (size_t)((long long)ks.base[0] + (long long)ia * ks.step[0] + (long long)ic * ks.step[1]);
+      off[1] = (size_t)((long long)ks.base[1] + (long long)ia * ks.step[2] + (long long)id * ks.step[3]);
+      off[2] = (size_t)((long long)ks.base[2] + (long long)ib * ks.step[4] + (long long)ic * ks.step[5]);
+      off[3] = (size_t)((long long)ks.base[3] + (long long)ib * ks.step[6] + (long long)id * ks.step[7]);
+   }
+   const double symm[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
+   const double fac = symm[(flags >> 4) & 7] * kt.hf_fac;
+   if (fac == 0.0) return;
+   const unsigned ldac = ld01 & 0xffff, ldad = ld01 >> 16, ldbc = ld23 & 0xffff, ldbd = ld23 >> 16;
+   const bool Tac = flags & 1, Tad = flags & 2, Tbc = flags & 4, Tbd = flags & 8;
+#pragma unroll
+   for (int isa = 0; isa < 2 * la + 1; isa++)
+#pragma unroll
+      for (int isb = 0; isb < 2 * lb + 1; isb++)
+#pragma unroll
+         for (int isc = 0; isc < 2 * lc + 1; isc++)
+#pragma unroll
+            for (int isd = 0; isd < 2 * ld + 1; isd++) {
+               const int tt = ((isa * (2 * lb + 1) + isb) * (2 * lc + 1) + isc) * (2 * ld + 1) + isd;
+               const size_t Iac = off[0] + (Tac ? isa * ldac + isc : isc * ldac + isa);
+               const size_t Iad = off[1] + (Tad ? isa * ldad + isd : isd * ldad + isa);
+               const size_t Ibc = off[2] + (Tbc ? isb * ldbc + isc : isc * ldbc + isb);
+               const size_t Ibd = off[3] + (Tbd ? isb * ldbd + isd : isd * ldbd + isb);
+               const double iabcd = -fac * s1[tt];
+               for (int sp = 0; sp < kt.nspin; sp++) {
+                  const double *P = kt.P[sp];
+                  double *K = kt.K[sp];
+                  unsafeAtomicAdd(&K[Ibd], iabcd * P[Iac]);
+                  unsafeAtomicAdd(&K[Ibc], iabcd * P[Iad]);
+                  unsafeAtomicAdd(&K[Iad], iabcd * P[Ibc]);
+                  unsafeAtomicAdd(&K[Iac], iabcd * P[Ibd]);
+               }
+            }
+}
+
+template <int la, int lb, int lc, int ld>
+int launch_one(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt,
+               hipStream_t st)
+{
+   TqArgs y;
+   const int sh = tp.cls_sh[cls_k];
+   y.sa = sh & 15;
+   y.sb = (sh >> 4) & 15;
+   y.sc = (sh >> 8) & 15;
+   y.sd = (sh >> 12) & 15;
+   y.nla = tp.nl[0][y.sa];
+   y.nlb = tp.nl[1][y.sb];
+   y.nlc = tp.nl[2][y.sc];
+   y.nld = tp.nl[3][y.sd];
+   y.nq_all = tp.cls_nl[cls_k];
+   y.nclass = tp.nclass;
+   for (int e = 0; e < 4; e++)
+      for (int f = 0; f < 4; f++)
+         y.bsrc[e][f] = (la + e <= la + lb && lc + f <= lc + ld) ? tp.cpoff[la + e][lc + f] : 0;
+   const size_t items = (size_t)(b.g1 - b.g0) * (size_t)y.nq_all;
+   const size_t grid = (items + 63) / 64;
+   if (grid == 0) return 0;
+   if (grid > 0x7fffffff) return -4;
+   hipLaunchKernelGGL((k_tq<la, lb, lc, ld>), dim3((unsigned)grid), dim3(64), 0, st, cls_k, t, b, kt, y);
+   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace tq
+
+// the classes with a thread-per-contraction kernel
+#define GINT_TQ_CLASSES(X)                                                                                      \
+   X(0, 0, 0, 0) X(1, 0, 0, 0) X(0, 0, 1, 0) X(0, 1, 0, 0) X(0, 0, 0, 1) X(2, 0, 0, 0) X(0, 0, 2, 0)          \
+   X(1, 1, 0, 0) X(1, 0, 1, 0) X(0, 0, 1, 1) X(0, 2, 0, 0) X(0, 0, 0, 2) X(1, 0, 0, 1) X(0, 1, 1, 0)          \
+   X(0, 1, 0, 1) X(2, 1, 0, 0) X(0, 0, 2, 1) X(2, 0, 1, 0) X(1, 0, 2, 0) X(1, 2, 0, 0) X(0, 0, 1, 2)          \
+   X(2, 0, 0, 1) X(0, 1, 2, 0) X(1, 1, 1, 0) X(1, 0, 1, 1) X(1, 0, 0, 2) X(0, 2, 1, 0) X(0, 2, 0, 1)          \
+   X(1, 1, 0, 1) X(0, 1, 1, 1) X(0, 1, 0, 2) X(2, 0, 2, 0) X(2, 2, 0, 0) X(0, 0, 2, 2)
+
+bool tq_has(int code)
+{
+   switch (code) {
+#define GINT_TQ_CASE(a, b, c, d) case class_code(a, b, c, d):
+      GINT_TQ_CLASSES(GINT_TQ_CASE)
+      return true;
+#undef GINT_TQ_CASE
+      default:
+         return false;
+   }
+}
+
+int launch_tq(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt, void *stream)
+{
+   if (b.g1 <= b.g0) return 0;
+   hipStream_t st = (hipStream_t)stream;
+   switch (tp.cls_code[cls_k]) {
+#define GINT_TQ_CASE(a, b_, c, d) \
+   case class_code(a, b_, c, d): return tq::launch_one<a, b_, c, d>(cls_k, tp, t, b, kt, st);
+      GINT_TQ_CLASSES(GINT_TQ_CASE)
+#undef GINT_TQ_CASE
+      default:
+         return -5;
+   }
+}
+
+}  // namespace gint

@@ -193,8 +193,10 @@ class Stream:
 
 
 def setup_handle(lib, prefix, system, hf_fac=1.0, max_mem_mib=4096, truncated=True, P=None, Pb=None,
-                 stats=False, timing=False, debug=0):
-    """create + configure a handle of `lib` (product or oracle) for `system`"""
+                 stats=False, timing=False, debug=0, c0=None):
+    """create + configure a handle of `lib` (product or oracle) for `system`;
+    c0 = (table, ld_C0), e.g. from a t_c_g.dat via libgint_amd.read_c0, in
+    place of the generated coefficient table"""
     f = lambda n: getattr(lib, prefix + n)
     h = f("create_handle")()
     f("init")(h)
@@ -205,7 +207,8 @@ def setup_handle(lib, prefix, system, hf_fac=1.0, max_mem_mib=4096, truncated=Tr
     if debug and prefix == "libgint_":
         lib.libgint_amd_set_debug(h, debug)
     if truncated:
-        c0, ld = tcg_table()
+        c0, ld = tcg_table() if c0 is None else c0
+        c0 = np.ascontiguousarray(c0, dtype=np.float64)
         f("set_Potential_Truncated")(h, system.rcut(), _ptr(c0.ravel()), ld, c0.size)
     f("set_hf_fac")(h, hf_fac)
     f("set_max_mem")(h, max_mem_mib)

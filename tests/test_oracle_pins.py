@@ -201,3 +201,28 @@ def test_truncated_coulomb_against_closed_form():
             assert np.max(np.abs(G - want)) < 3e-11, (p["R"], p["T"], np.max(np.abs(G - want)))
     lib.oracle_destroy_handle(h)
     assert seen == {0, 1, 2}
+
+
+@pytest.mark.parametrize("l", [0, 1, 2, 3])
+def test_norm_psi_by_quadrature(l):
+    """compute_norm_psi (reference src/extern_functions.cpp:95-110) makes a
+    contracted cartesian function whose exponents are all <= 1 (x^l for
+    l <= 1, xy, xyz) have unit norm; checked by 1-D quadrature of the
+    factorised overlap, independent of the closed form"""
+    from scipy.integrate import quad
+
+    lib = oracle()
+    rng = np.random.default_rng(40 + l)
+    Z = 10 ** rng.uniform(-1, 2, 4)
+    K = rng.normal(size=4)
+    N = lib.oracle_norm_psi(as_dp(K), as_dp(Z), 4, l)
+    powers = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 1, 1)][l]
+    S = 0.0
+    for i in range(4):
+        for j in range(4):
+            a = Z[i] + Z[j]
+            term = K[i] * K[j]
+            for p in powers:
+                term *= quad(lambda x: x ** (2 * p) * math.exp(-a * x * x), -np.inf, np.inf, epsabs=0, epsrel=1e-13)[0]
+            S += term
+    assert abs(S / N ** 2 - 1.0) < 1e-10
