@@ -35,12 +35,26 @@ def main():
     ap.add_argument("--cpu-sample-sets", type=int, default=0, help="sets in the CPU sample (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--c0-file", default=None, help="CP2K t_c_g.dat to use in place of the generated table")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 3 device copies of MIB MiB (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--max-mem", type=int, default=65536, help="device budget, MiB")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+
+    if a.copy_probe:
+        import torch
+
+        n = a.copy_probe * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 8 * n, "copy_probe_bytes_written": 8 * n, "launches": 3}))
+        return
 
     from libgint_amd import driver as D, load, read_c0
 
@@ -148,7 +162,7 @@ def main():
         "value": round(value, 1), "unit": "quartets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(1e3 * tot / a.steps, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"config {a.config}", "atoms": s.natom, "nbf": s.nbf, "cells": s.Ng,
+        "config": {"workload": s.describe(), "atoms": s.natom, "nbf": s.nbf, "cells": s.Ng,
                    "set_quartets": st.n_sets, "primitive_quartets": st.n_prm, "contracted_quartets": st.n_qrtt,
                    "surviving_slots": int(slots), "abi_ingestion_s": round(t_ing, 2),
                    "path": "FP64, resident plan, fused Hermite cell sum per quartet type"},

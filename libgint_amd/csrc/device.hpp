@@ -152,8 +152,8 @@ struct DevTables {
 // from the chunk's first primitive and first row.
 struct BatchDev {
    const TypePlan *tp;        // device copy
-   const uint32_t *e1;        // Hermite -> cartesian stage 1 items: t u v fx fy fz, 4 bits each
-   const uint32_t *e2;        // stage 2 items: ex ey ez (4 bits each), column << 12
+   const uint32_t *e1;        // Hermite -> cartesian stage 1 items: t u v fx fy fz (3 bits each), W slot << 18
+   const uint32_t *e2;        // stage 2 items: ex ey ez (3 bits each), column << 9, block element << 16
    const RowRec *rows;
    const GroupRec *groups;
    const uint32_t *prm_row;   // row of each primitive quartet

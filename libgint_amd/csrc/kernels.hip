@@ -314,7 +314,6 @@ struct FmLayout {
    int o_sv;    // per lane: X Y Z of P-Q-g, TG1 (or T), TG2
    int o_R;     // Hermite layers, 2 x SB x nh
    int o_int;   // byte offset of 64 lane infos (branch | patch << 2) and 64 survivor lanes
-   int o_desc;  // byte offset of the transform item descriptors (e1 then e2), 0 if they stay in HBM
    int bytes;
 };
 
@@ -336,13 +335,6 @@ static FmLayout fm_layout(const TypePlan &tp, bool fix)
    const int dbl = std::max(cell_phase, e_phase);
    y.o_int = 8 * dbl;
    y.bytes = y.o_int + 4 * 128;
-   // the transform's item descriptors, once per workgroup, when small
-   const int ndesc = tp.nbh * tp.nkc + tp.ncp;
-   y.o_desc = 0;
-   if (!fix && 4 * ndesc <= 12288) {
-      y.o_desc = y.bytes;
-      y.bytes += 4 * ndesc;
-   }
    return y;
 }
 
@@ -361,16 +353,7 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
    int *svi = (int *)((char *)lds + lay.o_int);
    int *svl = svi + 64;
    const uint32_t nwork = FIX ? min(b.fix_count[0], b.fix_cap) : (b.p1 - b.p0);
-   const uint32_t *e1 = b.e1, *e2 = b.e2;
-   if (lay.o_desc) {
-      uint32_t *d = (uint32_t *)((char *)lds + lay.o_desc);
-      const int n1 = tp.nbh * tp.nkc;
-      for (int i = threadIdx.x; i < n1; i += 64) d[i] = b.e1[i];
-      for (int i = threadIdx.x; i < tp.ncp; i += 64) d[n1 + i] = b.e2[i];
-      e1 = d;
-      e2 = d + n1;
-      __syncthreads();
-   }
+   const uint32_t *__restrict__ e1 = b.e1, *__restrict__ e2 = b.e2;
 
    for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
       uint32_t p;
@@ -651,8 +634,8 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
          // W[tuv][f] = sum E^f_tau E^f_nu E^f_phi R[t+tau, u+nu, v+phi]
          for (int idx = lane; idx < tp.nbh * nkc; idx += 64) {
             const uint32_t it = e1[idx];
-            const int tt = it & 15, uu = (it >> 4) & 15, vv = (it >> 8) & 15;
-            const int fx = (it >> 12) & 15, fy = (it >> 16) & 15, fz = (it >> 20) & 15;
+            const int tt = it & 7, uu = (it >> 3) & 7, vv = (it >> 6) & 7;
+            const int fx = (it >> 9) & 7, fy = (it >> 12) & 7, fz = (it >> 15) & 7;
             const double *Fx = Fk + fx * (fx + 1) / 2, *Fy = Fk + 28 + fy * (fy + 1) / 2, *Fz = Fk + 56 + fz * (fz + 1) / 2;
             double s = 0.0;
             for (int a = 0; a <= fx; a++)
@@ -665,13 +648,13 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
                      s += cxy * Fz[d] * sR[H];
                   }
                }
-            W[idx] = s;
+            W[it >> 18] = s;
          }
          __syncthreads();
          // (e0|f0) = sum E^e_t E^e_u E^e_v W[tuv][f]
          for (int j = lane; j < tp.ncp; j += 64) {
             const uint32_t it = e2[j];
-            const int ex = it & 15, ey = (it >> 4) & 15, ez = (it >> 8) & 15, col = (int)(it >> 12);
+            const int ex = it & 7, ey = (it >> 3) & 7, ez = (it >> 6) & 7, col = (int)((it >> 9) & 127);
             const double *Ea = Ex + ex * (ex + 1) / 2, *Eb = Ex + 28 + ey * (ey + 1) / 2, *Ec = Ex + 56 + ez * (ez + 1) / 2;
             double s = 0.0;
             for (int a = 0; a <= ex; a++)
@@ -683,8 +666,9 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
                      s += cxy * Ec[d] * W[H * nkc + col];
                   }
                }
-            if (!FIX && j == 0 && tp.has00) s += quirk;
-            dst[j] = s;
+            const int jd = (int)(it >> 16);
+            if (!FIX && jd == 0 && tp.has00) s += quirk;
+            dst[jd] = s;
          }
          __syncthreads();
       }
@@ -718,7 +702,6 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
 // quartet's contractions before the atomics when the spherical integrals of
 // all contractions fit on chip.
 // ---------------------------------------------------------------------------
-constexpr int CQ_THREADS = 128;
 constexpr int CPL_BYTES = (sizeof(ClassPlan) + 15) & ~15;   // LDS copy of the class plan
 constexpr int CQ_PCH = 16;   // primitive quartets per contraction step
 
@@ -771,6 +754,7 @@ __device__ __forceinline__ unsigned blk_idx(bool T, unsigned ld, unsigned row_i,
 }
 
 // one (set quartet, class): the work of k_gqs for class cls_k of group gi
+template <int NT>
 __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b,
                        const KTargets &kt, const CqLayout &y, double *WS, const int *__restrict__ srcmap, uint32_t gi,
                        const double *co, uint32_t *ipb)
@@ -801,14 +785,14 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
    for (int l0 = 0; l0 < nq; l0 += y.lb) {
       const int nb = min(y.lb, nq - l0);
       __syncthreads();
-      for (int idx = tid; idx < nb * Nc; idx += CQ_THREADS) {
+      for (int idx = tid; idx < nb * Nc; idx += NT) {
          const int l = fdiv(idx, Nc);
          WS[l * lws + 3 * cap + (idx - l * Nc)] = 0.0;
       }
       for (uint32_t r = g.row0; r < g.row1; r++) {
          const RowRec row = b.rows[r];
          // ---- contraction of the row's primitive quartets into buf0 ----
-         for (int idx = tid; idx < nb * ncp; idx += CQ_THREADS) {
+         for (int idx = tid; idx < nb * ncp; idx += NT) {
             const int l = fdiv(idx, ncp);
             WS[l * lws + (idx - l * ncp)] = 0.0;
          }
@@ -819,13 +803,13 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
             __syncthreads();
             if (tid < np) ipb[tid] = IP[pc + tid];
             __syncthreads();
-            for (int idx = tid; idx < nb * CQ_PCH; idx += CQ_THREADS) {
+            for (int idx = tid; idx < nb * CQ_PCH; idx += NT) {
                const int l = idx / CQ_PCH, pp = idx % CQ_PCH;
                Kp[idx] = pp < np ? kprod_lds(tp, co, cls_k, l0 + l, ipb[pp]) : 0.0;
             }
             __syncthreads();
             const double *A = A0 + (size_t)pc * ncpt;
-            for (int idx = tid; idx < nb * ncp; idx += CQ_THREADS) {
+            for (int idx = tid; idx < nb * ncp; idx += NT) {
                const int l = fdiv(idx, ncp), j = idx - l * ncp;
                const int src = srcmap[j];
                const double *kp = Kp + l * CQ_PCH;
@@ -846,13 +830,13 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
                const double *src = b.fix_ws + (size_t)e * (ncpt + 16);
                const uint32_t ip = b.prm_ip[p];
                if (keep) {
-                  for (int idx = tid; idx < nb * ncp; idx += CQ_THREADS) {
+                  for (int idx = tid; idx < nb * ncp; idx += NT) {
                      const int l = fdiv(idx, ncp), j = idx - l * ncp;
                      WS[l * lws + j] += kprod(tp, g, b.data, cls_k, l0 + l, ip) * src[srcmap[j]];
                   }
                } else if (la == 0 && lc == 0) {
                   // a screened slot keeps its Fm[0] (reference src/compute_VRR.cpp:384-385)
-                  for (int l = tid; l < nb; l += CQ_THREADS)
+                  for (int l = tid; l < nb; l += NT)
                      WS[l * lws] += kprod(tp, g, b.data, cls_k, l0 + l, ip) * src[ncpt + gate];
                }
                __syncthreads();
@@ -867,7 +851,7 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
             const int dst = (srcb + 1) % 3;
             const int nd = d_nco(dd), ndm = d_nco(dd - 1);
             const int sz = cp.ksize[dd];
-            for (int idx1 = tid; idx1 < nb * sz; idx1 += CQ_THREADS) {
+            for (int idx1 = tid; idx1 < nb * sz; idx1 += NT) {
                const int l = fdiv(idx1, sz), idx0 = idx1 - l * sz;
                const double *in = WS + l * lws + srcb * cap;
                int e = la, c = lc;
@@ -899,7 +883,7 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
             const int dst = (srcb + 1) % 3;
             const int nbb = d_nco(bb), nbm = d_nco(bb - 1);
             const int sz = cp.bsize[bb];
-            for (int idx1 = tid; idx1 < nb * sz; idx1 += CQ_THREADS) {
+            for (int idx1 = tid; idx1 < nb * sz; idx1 += NT) {
                const int l = fdiv(idx1, sz), idx0 = idx1 - l * sz;
                const double *in = WS + l * lws + srcb * cap;
                int a = la;
@@ -919,7 +903,7 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
             srcb = dst;
          }
          // the first block of the last layer is (la lb|lc ld)
-         for (int idx = tid; idx < nb * Nc; idx += CQ_THREADS) {
+         for (int idx = tid; idx < nb * Nc; idx += NT) {
             const int l = fdiv(idx, Nc), i = idx - l * Nc;
             WS[l * lws + 3 * cap + i] += WS[l * lws + srcb * cap + i];
          }
@@ -936,7 +920,7 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
             const int n = dims[1] * dims[2] * dims[3];
             const int m = 2 * ls[s] + 1;
             const double *c2s = c_c2s + c_c2s_off[ls[s]];
-            for (int idx1 = tid; idx1 < nb * n * m; idx1 += CQ_THREADS) {
+            for (int idx1 = tid; idx1 < nb * n * m; idx1 += NT) {
                const int l = fdiv(idx1, n * m), idx = idx1 - l * n * m;
                const int j = idx / m, i = idx - j * m;
                const double *in = WS + l * lws + in_off;
@@ -955,14 +939,14 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
       }
       // spherical integrals of contraction l0+l at WS + l lws + cap
       if (reduce) {
-         for (int idx = tid; idx < nb * Ns; idx += CQ_THREADS) {
+         for (int idx = tid; idx < nb * Ns; idx += NT) {
             const int l = fdiv(idx, Ns);
             Iall[(l0 + l) * Ns + (idx - l * Ns)] = WS[l * lws + cap + (idx - l * Ns)];
          }
          continue;
       }
       // one contraction at a time: 4 atomic updates per spherical integral
-      for (int idx = tid; idx < nb * Ns; idx += CQ_THREADS) {
+      for (int idx = tid; idx < nb * Ns; idx += NT) {
          const int l = fdiv(idx, Ns), tt = idx - l * Ns;
          size_t off[4];
          uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
@@ -1016,7 +1000,7 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
    for (int sp = 0; sp < kt.nspin; sp++) {
       const double *P = kt.P[sp];
       double *K = kt.K[sp];
-      for (int x = tid; x < NB * ND; x += CQ_THREADS) {  // K_bd
+      for (int x = tid; x < NB * ND; x += NT) {  // K_bd
          const int ib = (x / ND) / nsb, isb = (x / ND) % nsb, id = (x % ND) / nsd, isd = (x % ND) % nsd;
          double sum = 0.0;
          for (int ia = 0; ia < nla; ia++)
@@ -1028,7 +1012,7 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
             }
          unsafeAtomicAdd(&K[(size_t)offs(0, ib, 0, id, 3) + blk_idx(Tbd, ldbd, isb, isd)], -fac * sum);
       }
-      for (int x = tid; x < NB * NC; x += CQ_THREADS) {  // K_bc
+      for (int x = tid; x < NB * NC; x += NT) {  // K_bc
          const int ib = (x / NC) / nsb, isb = (x / NC) % nsb, ic = (x % NC) / nsc, isc = (x % NC) % nsc;
          double sum = 0.0;
          for (int ia = 0; ia < nla; ia++)
@@ -1040,7 +1024,7 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
             }
          unsafeAtomicAdd(&K[(size_t)offs(0, ib, ic, 0, 2) + blk_idx(Tbc, ldbc, isb, isc)], -fac * sum);
       }
-      for (int x = tid; x < NA * ND; x += CQ_THREADS) {  // K_ad
+      for (int x = tid; x < NA * ND; x += NT) {  // K_ad
          const int ia = (x / ND) / nsa, isa = (x / ND) % nsa, id = (x % ND) / nsd, isd = (x % ND) % nsd;
          double sum = 0.0;
          for (int ib = 0; ib < nlb; ib++)
@@ -1052,7 +1036,7 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
             }
          unsafeAtomicAdd(&K[(size_t)offs(ia, 0, 0, id, 1) + blk_idx(Tad, ldad, isa, isd)], -fac * sum);
       }
-      for (int x = tid; x < NA * NC; x += CQ_THREADS) {  // K_ac
+      for (int x = tid; x < NA * NC; x += NT) {  // K_ac
          const int ia = (x / NC) / nsa, isa = (x / NC) % nsa, ic = (x % NC) / nsc, isc = (x % NC) % nsc;
          double sum = 0.0;
          for (int ib = 0; ib < nlb; ib++)
@@ -1069,7 +1053,8 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
 
 // one workgroup per set quartet, its classes in turn: the group's rows,
 // primitive lists and coefficients stay hot in the caches across classes
-__global__ __launch_bounds__(CQ_THREADS) void k_gqs(const ClassPlan *__restrict__ cps, const CqLayout *__restrict__ lays,
+template <int NT>
+__global__ __launch_bounds__(NT) void k_gqs(const ClassPlan *__restrict__ cps, const CqLayout *__restrict__ lays,
                                                     const int *__restrict__ srcmaps, const TypePlan *__restrict__ tpp,
                                                     DevTables t, BatchDev b, KTargets kt, double *gws, int gws_stride,
                                                     int cls_bytes)
@@ -1088,7 +1073,7 @@ __global__ __launch_bounds__(CQ_THREADS) void k_gqs(const ClassPlan *__restrict_
          const int npt = ((g.np4 >> (4 * tc)) & 15) + 1;
          for (int s = 0; s < tp.nsh[tc]; s++) {
             const int n = tp.nl[tc][s] * MAXNP;
-            for (int i = tid; i < n; i += CQ_THREADS) {
+            for (int i = tid; i < n; i += NT) {
                const int inl = i / MAXNP, ip = i % MAXNP;
                co[tp.co_off[tc][s] + i] = ip < npt ? b.data[g.Kof[tc][s] + inl * npt + ip] : 0.0;
             }
@@ -1101,11 +1086,11 @@ __global__ __launch_bounds__(CQ_THREADS) void k_gqs(const ClassPlan *__restrict_
          {
             const int *src = (const int *)(cps + k);
             int *dst = (int *)cpl;
-            for (int i = tid; i < (int)(sizeof(ClassPlan) / 4); i += CQ_THREADS) dst[i] = src[i];
+            for (int i = tid; i < (int)(sizeof(ClassPlan) / 4); i += NT) dst[i] = src[i];
          }
          __syncthreads();
          double *WS = y.ws_global ? gws + (size_t)blockIdx.x * gws_stride : lds_base() + y.o_ws;
-         cq_one(*cpl, k, tp, t, b, kt, y, WS, srcmaps + y.o_src, gi, co, ipb);
+         cq_one<NT>(*cpl, k, tp, t, b, kt, y, WS, srcmaps + y.o_src, gi, co, ipb);
       }
       __syncthreads();
    }
@@ -1127,8 +1112,7 @@ int launch_fm_r(const TypePlan &tp, const DevTables &t, const BatchDev &b, void 
    if (b.p1 <= b.p0) return 0;
    const FmLayout y = fm_layout(tp, false);
    if ((size_t)y.bytes > LDS_LIMIT) return -3;
-   // workgroups stay resident and stride over the primitive quartets, so the
-   // descriptors staged in LDS serve many of them
+   // workgroups stay resident and stride over the primitive quartets
    const unsigned grid = std::min(b.p1 - b.p0, 16384u);
    hipLaunchKernelGGL(k_fm_r<false>, dim3(grid), dim3(64), y.bytes, (hipStream_t)stream, b.tp, t, b, y);
    return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -1214,8 +1198,14 @@ int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lay
       grid = std::min(grid, SCRATCH_GRID);
       if (!s.ptr || s.doubles < (size_t)gws_stride * grid) return -2;
    }
-   hipLaunchKernelGGL(k_gqs, dim3(grid), dim3(CQ_THREADS), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
-                      b.tp, t, b, kt, s.ptr, gws_stride, cls_bytes);
+   // a type whose workspace leaves room for few workgroups per CU gets wider
+   // ones, so that more wavefronts hide the memory latency
+   if (total > 32 * 1024)
+      hipLaunchKernelGGL(k_gqs<256>, dim3(grid), dim3(256), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
+                         b.tp, t, b, kt, s.ptr, gws_stride, cls_bytes);
+   else
+      hipLaunchKernelGGL(k_gqs<128>, dim3(grid), dim3(128), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
+                         b.tp, t, b, kt, s.ptr, gws_stride, cls_bytes);
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

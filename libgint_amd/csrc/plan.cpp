@@ -272,24 +272,32 @@ void make_type_items(const TypePlan &tp, std::vector<uint32_t> &e1, std::vector<
          for (int ly = L - lx; ly >= 0; ly--) v.push_back({lx, ly, L - lx - ly});
       return v;
    };
-   std::vector<std::array<int, 3>> ket;
+   std::vector<std::array<int, 3>> ket, bra;
    for (int f = tp.LC0; f <= tp.LC; f++)
       for (auto &c : comps(f)) ket.push_back(c);
-   e1.clear();
    for (int N = 0; N <= tp.LA; N++)
-      for (auto &h : comps(N))
-         for (auto &c : ket)
-            e1.push_back((uint32_t)h[0] | (uint32_t)h[1] << 4 | (uint32_t)h[2] << 8 | (uint32_t)c[0] << 12 |
-                         (uint32_t)c[1] << 16 | (uint32_t)c[2] << 20);
-   e2.assign(tp.ncp, 0u);
+      for (auto &h : comps(N)) bra.push_back(h);
+   // stage 1, W[h][kc] = sum over the ket E of R: ket component outer, so the
+   // lanes of a wavefront share their loop bounds (fx, fy, fz)
+   e1.clear();
+   const int nkc = (int)ket.size();
+   for (int kc = 0; kc < nkc; kc++)
+      for (int h = 0; h < (int)bra.size(); h++) {
+         const auto &t = bra[h], &c = ket[kc];
+         e1.push_back((uint32_t)t[0] | (uint32_t)t[1] << 3 | (uint32_t)t[2] << 6 | (uint32_t)c[0] << 9 |
+                      (uint32_t)c[1] << 12 | (uint32_t)c[2] << 15 | (uint32_t)(h * nkc + kc) << 18);
+      }
+   // stage 2, (e0|f0) = sum over the bra E of W: bra component outer for the
+   // same reason; each item names its destination j in the type's blocks
+   e2.clear();
    for (int e = tp.LA0; e <= tp.LA; e++) {
       const auto ce = comps(e);
-      for (int f = tp.LC0; f <= tp.LC; f++)
-         for (int ie = 0; ie < nco(e); ie++)
+      for (int ie = 0; ie < nco(e); ie++)
+         for (int f = tp.LC0; f <= tp.LC; f++)
             for (int jl = 0; jl < nco(f); jl++) {
                const int j = tp.cpoff[e][f] + ie * nco(f) + jl;
-               e2[j] = (uint32_t)ce[ie][0] | (uint32_t)ce[ie][1] << 4 | (uint32_t)ce[ie][2] << 8 |
-                       (uint32_t)(tp.kcoff[f] + jl) << 12;
+               e2.push_back((uint32_t)ce[ie][0] | (uint32_t)ce[ie][1] << 3 | (uint32_t)ce[ie][2] << 6 |
+                            (uint32_t)(tp.kcoff[f] + jl) << 9 | (uint32_t)j << 16);
             }
    }
 }

@@ -112,6 +112,15 @@ class System:
         self.natom, self.nbf, self.P_size, self.Ng = info[0], info[1], info[2], info[3]
         self.np_kinds = (info[5], info[6], info[7])
 
+    def describe(self):
+        """the workload in words, with its sizes"""
+        p = self.params
+        basis = {0: "O 3s2p1d / H 2s1p", 1: "O 2s1p / H 1s", 4: "O 4s3p2d1f / H 3s2p1d"}[p["basis"]]
+        tm = f" + {p['n_tm']} spdf centres" if p["n_tm"] else ""
+        box = f"{p['box_A']} A periodic box" if self.periodic else f"{p['box_A']} A cluster"
+        return (f"{p['n_water']} H2O{tm} in a {box}, {self.Ng} cells, {basis}, "
+                f"truncated Coulomb R_c {p['rcut_A']} A, {self.nbf} basis functions")
+
     def __del__(self):
         try:
             drv().gdrv_system_free(self.h)
