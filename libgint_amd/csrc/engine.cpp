@@ -146,6 +146,10 @@ Engine::~Engine()
    for (void *b : bufs)
       if (b) (void)hipFree(b);
    for (void *e : ev_pool_) (void)hipEventDestroy((hipEvent_t)e);
+   for (void *e : ev_join_)
+      if (e) (void)hipEventDestroy((hipEvent_t)e);
+   for (void *q : side_)
+      if (q) (void)hipStreamDestroy((hipStream_t)q);
 }
 
 void Engine::init()
@@ -159,6 +163,15 @@ void Engine::init()
    if (my_thr_ >= (int)g_streams.size()) fatal("handle created outside the team that calls init", my_thr_);
    if (!g_streams[my_thr_]) HIPCHECK(hipStreamCreateWithFlags(&g_streams[my_thr_], hipStreamNonBlocking));
    stream_ = g_streams[my_thr_];
+   ensure_side();
+}
+
+void Engine::ensure_side()
+{
+   for (int i = 0; i < NSIDE; i++)
+      if (!side_[i]) HIPCHECK(hipStreamCreateWithFlags((hipStream_t *)&side_[i], hipStreamNonBlocking));
+   for (int i = 0; i <= NSIDE; i++)
+      if (!ev_join_[i]) HIPCHECK(hipEventCreateWithFlags((hipEvent_t *)&ev_join_[i], hipEventDisableTiming));
 }
 
 void Engine::set_potential_truncated(double R_cut, const double *C0, int ld_C0, int C0_size)
@@ -818,7 +831,7 @@ void Engine::ensure_work(size_t ncp_max, size_t max_prm_chunk)
    }
 }
 
-void Engine::ev_begin(int cat)
+void Engine::ev_begin(int cat, void *stream)
 {
    (void)cat;
    if (!timing_on_) return;
@@ -831,14 +844,14 @@ void Engine::ev_begin(int cat)
       ev_pool_.push_back(e1);
    }
    ev_open_ = ev_pool_[ev_used_];
-   HIPCHECK(hipEventRecord((hipEvent_t)ev_open_, (hipStream_t)stream_));
+   HIPCHECK(hipEventRecord((hipEvent_t)ev_open_, (hipStream_t)stream));
 }
 
-void Engine::ev_end(int cat)
+void Engine::ev_end(int cat, void *stream)
 {
    if (!timing_on_) return;
    void *e1 = ev_pool_[ev_used_ + 1];
-   HIPCHECK(hipEventRecord((hipEvent_t)e1, (hipStream_t)stream_));
+   HIPCHECK(hipEventRecord((hipEvent_t)e1, (hipStream_t)stream));
    ev_live_.push_back(EvRec{cat, ev_open_, e1, ev_key_});
    ev_used_ += 2;
 }
@@ -907,32 +920,41 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
    }
 
    HIPCHECK(hipMemsetAsync(d_fixcnt_, 0, sizeof(uint32_t), st));
-   ev_begin(0);
+   ev_begin(0, st);
    ev_key_ = tp.L;
    int rc = launch_fm_r(tp, t, b, st);
    if (rc) fatal("fm_r launch", rc);
-   ev_end(0);
-   ev_begin(3);
+   ev_end(0, st);
+   ev_begin(3, st);
    rc = launch_fixup(tp, t, b, (debug_ & 1) ? (uint32_t)fix_cap_ : 64u, st);
    if (rc) fatal("fix-up launch", rc);
-   ev_end(3);
-   ev_begin(1);
-   ev_key_ = tp.L;
-   for (int k = 0; k < tp.nclass; k++) {
-      if (!td.class_used[k] || !ti.lays[k].skip) continue;
-      rc = launch_tq(k, tp, t, b, kt, st);
-      if (rc) fatal("tq launch", rc);
-   }
-   ev_end(1);
+   ev_end(3, st);
+   ensure_side();
+   HIPCHECK(hipEventRecord((hipEvent_t)ev_join_[0], st));
+   for (int i = 0; i < NSIDE; i++) HIPCHECK(hipStreamWaitEvent((hipStream_t)side_[i], (hipEvent_t)ev_join_[0], 0));
    bool any_gqs = false;
    for (int k = 0; k < tp.nclass; k++) any_gqs = any_gqs || (td.class_used[k] && !ti.lays[k].skip);
    if (any_gqs) {
-      ev_begin(2);
+      ev_begin(2, st);
       ev_key_ = tp.L;
       rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, t, b, kt, Scratch{d_scratch_, scratch_cap_},
                       st);
       if (rc) fatal("gqs launch", rc);
-      ev_end(2);
+      ev_end(2, st);
+   }
+   int si = 0;
+   for (int k = 0; k < tp.nclass; k++) {
+      if (!td.class_used[k] || !ti.lays[k].skip) continue;
+      void *q = side_[si++ % NSIDE];
+      ev_begin(1, q);
+      ev_key_ = tp.L;
+      rc = launch_tq(k, tp, t, b, kt, q);
+      if (rc) fatal("tq launch", rc);
+      ev_end(1, q);
+   }
+   for (int i = 0; i < NSIDE; i++) {
+      HIPCHECK(hipEventRecord((hipEvent_t)ev_join_[1 + i], (hipStream_t)side_[i]));
+      HIPCHECK(hipStreamWaitEvent(st, (hipEvent_t)ev_join_[1 + i], 0));
    }
    timing_.flops[1] += c.tq_flops;
    timing_.flops[2] += c.k3_flops;

@@ -154,10 +154,17 @@ class Engine {
    void ensure_tables();
    void upload_data();
    void harvest_events();
-   void ev_begin(int cat);
-   void ev_end(int cat);
+   void ev_begin(int cat, void *stream);
+   void ev_end(int cat, void *stream);
+   void ensure_side();
    int my_thr_ = 0, nomp_ = 1;
    void *stream_ = nullptr;
+   // the consumers of a chunk's blocks only add into K: the thread-per-
+   // contraction kernels run on side streams beside gqs on stream_, and
+   // stream_ waits for them before the blocks are overwritten
+   static constexpr int NSIDE = 3;
+   void *side_[NSIDE] = {};
+   void *ev_join_[NSIDE + 1] = {};   // [0] blocks ready, [1 + i] side stream i done
 
    // geometry, potential
    double cell_[18];
