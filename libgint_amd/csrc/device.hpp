@@ -198,7 +198,8 @@ struct CqLayout {
    int o_src;       // first entry of the class's block map (class element -> type element) in the type's maps
    int reduce;
    int ws_global;   // workspace in HBM (gws + blockIdx * lb * lws)
-   int skip;        // the class runs in its thread-per-contraction kernel instead
+   int skip;        // the class runs in its own templated kernel instead
+   int part;        // which of the GQS_PARTS concurrent launches takes the class
    int bytes;
 };
 
@@ -207,12 +208,17 @@ int launch_fm_r(const TypePlan &tp, const DevTables &t, const BatchDev &b, void 
 int launch_fixup(const TypePlan &tp, const DevTables &t, const BatchDev &b, uint32_t n_fix, void *stream);
 // thread-per-contraction kernels of the small classes (kernels_tq.hip)
 bool tq_has(int code);
-int launch_tq(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt, void *stream);
+int launch_tq(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt,
+              const int *srcmap, void *stream);
 // per-class LDS plans and block maps of a type for the quartet kernel
 void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::vector<CqLayout> &lays,
                   std::vector<int> &srcmaps, int *lds_bytes, int *gws_stride);
+// the classes of a type that k_gqs takes are split into this many launches,
+// balanced by their estimated cost, which run concurrently on separate streams
+constexpr int GQS_PARTS = 4;
 int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lays, const int *d_srcmaps, int lds_bytes,
-               int gws_stride, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s, void *stream);
+               int gws_stride, int part, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s,
+               void *stream);
 // LDS bytes of the fused kernel for a type (0 if it does not fit)
 size_t fm_r_lds_bytes(const TypePlan &tp);
 

@@ -929,26 +929,32 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
    rc = launch_fixup(tp, t, b, (debug_ & 1) ? (uint32_t)fix_cap_ : 64u, st);
    if (rc) fatal("fix-up launch", rc);
    ev_end(3, st);
+   // debug bit 1: every consumer on the main stream, one after the other
+   // (kernel durations without overlap, for profiling)
+   const bool serial = (debug_ & 2) != 0;
    ensure_side();
    HIPCHECK(hipEventRecord((hipEvent_t)ev_join_[0], st));
    for (int i = 0; i < NSIDE; i++) HIPCHECK(hipStreamWaitEvent((hipStream_t)side_[i], (hipEvent_t)ev_join_[0], 0));
-   bool any_gqs = false;
-   for (int k = 0; k < tp.nclass; k++) any_gqs = any_gqs || (td.class_used[k] && !ti.lays[k].skip);
-   if (any_gqs) {
-      ev_begin(2, st);
+   for (int part = 0; part < GQS_PARTS; part++) {
+      bool any = false;
+      for (int k = 0; k < tp.nclass; k++)
+         any = any || (td.class_used[k] && !ti.lays[k].skip && ti.lays[k].part == part);
+      if (!any) continue;
+      void *q = (part == 0 || serial) ? (void *)st : side_[(part - 1) % NSIDE];
+      ev_begin(2, q);
       ev_key_ = tp.L;
-      rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, t, b, kt, Scratch{d_scratch_, scratch_cap_},
-                      st);
+      rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, part, t, b, kt,
+                      Scratch{d_scratch_, scratch_cap_}, q);
       if (rc) fatal("gqs launch", rc);
-      ev_end(2, st);
+      ev_end(2, q);
    }
    int si = 0;
    for (int k = 0; k < tp.nclass; k++) {
       if (!td.class_used[k] || !ti.lays[k].skip) continue;
-      void *q = side_[si++ % NSIDE];
+      void *q = serial ? (void *)st : side_[si++ % NSIDE];
       ev_begin(1, q);
       ev_key_ = tp.L;
-      rc = launch_tq(k, tp, t, b, kt, q);
+      rc = launch_tq(k, tp, t, b, kt, ti.d_srcmaps + ti.lays[k].o_src, q);
       if (rc) fatal("tq launch", rc);
       ev_end(1, q);
    }
@@ -973,7 +979,7 @@ void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
       const TypeInfo &ti = types_[D.type];
       ncp_max = std::max(ncp_max, (size_t)ti.plan.ncp);
       for (auto &c : D.chunks) max_prm = std::max(max_prm, (size_t)(c.p1 - c.p0));
-      scr = std::max(scr, (size_t)ti.cq_gws * SCRATCH_GRID);
+      scr = std::max(scr, (size_t)ti.cq_gws * SCRATCH_GRID * GQS_PARTS);
    }
    if (ncp_max == 0) return;
    ensure_work(ncp_max, max_prm);

@@ -703,7 +703,7 @@ __global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, D
 // all contractions fit on chip.
 // ---------------------------------------------------------------------------
 constexpr int CPL_BYTES = (sizeof(ClassPlan) + 15) & ~15;   // LDS copy of the class plan
-constexpr int CQ_PCH = 16;   // primitive quartets per contraction step
+constexpr int CQ_PCH = 8;    // primitive quartets per contraction step
 
 
 __device__ __forceinline__ double *lds_base()
@@ -1053,11 +1053,13 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
 
 // one workgroup per set quartet, its classes in turn: the group's rows,
 // primitive lists and coefficients stay hot in the caches across classes
+// at most 128 VGPRs (4 wavefronts per SIMD): the kernel waits on memory, and
+// more wavefronts in flight outweigh the few values it then keeps in scratch
 template <int NT>
-__global__ __launch_bounds__(NT) void k_gqs(const ClassPlan *__restrict__ cps, const CqLayout *__restrict__ lays,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_gqs(const ClassPlan *__restrict__ cps, const CqLayout *__restrict__ lays,
                                                     const int *__restrict__ srcmaps, const TypePlan *__restrict__ tpp,
                                                     DevTables t, BatchDev b, KTargets kt, double *gws, int gws_stride,
-                                                    int cls_bytes)
+                                                    int cls_bytes, int part)
 {
    const TypePlan &tp = *tpp;
    const int tid = threadIdx.x;
@@ -1080,7 +1082,7 @@ __global__ __launch_bounds__(NT) void k_gqs(const ClassPlan *__restrict__ cps, c
          }
       }
       for (int k = 0; k < tp.nclass; k++) {
-         if ((ksrow[k].flags >> 16) == 0 || lays[k].skip) continue;
+         if ((ksrow[k].flags >> 16) == 0 || lays[k].skip || lays[k].part != part) continue;
          const CqLayout &y = lays[k];
          __syncthreads();
          {
@@ -1181,12 +1183,30 @@ void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::ve
          *lds_bytes = std::max(*lds_bytes, y.bytes);
          if (y.ws_global) *gws_stride = std::max(*gws_stride, y.lws);
       }
+      y.part = 0;
       lays.push_back(y);
+   }
+   // the parts: largest estimated cost first, each to the lightest part
+   std::vector<std::pair<double, int>> cost;
+   for (int k = 0; k < tp.nclass; k++)
+      if (!lays[k].skip)
+         cost.push_back({(double)tp.cls_nl[k] * (2.0 * cps[k].ncp + cps[k].hrr_flops + cps[k].sph_flops + 8.0 * cps[k].Ns), k});
+   std::sort(cost.begin(), cost.end(), [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
+      return a.first > b.first || (a.first == b.first && a.second < b.second);
+   });
+   double load[GQS_PARTS] = {};
+   for (const auto &c : cost) {
+      int best = 0;
+      for (int q = 1; q < GQS_PARTS; q++)
+         if (load[q] < load[best]) best = q;
+      lays[c.second].part = best;
+      load[best] += c.first;
    }
 }
 
 int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lays, const int *d_srcmaps, int lds_bytes,
-               int gws_stride, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s, void *stream)
+               int gws_stride, int part, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s,
+               void *stream)
 {
    if (b.g1 <= b.g0) return 0;
    const int cls_bytes = (lds_bytes + 15) & ~15;
@@ -1194,18 +1214,21 @@ int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lay
    if (total > LDS_LIMIT) return -3;
    const uint32_t ng = b.g1 - b.g0;
    unsigned grid = std::min(ng, 1u << 20);
+   double *gws = nullptr;
    if (gws_stride) {
+      // each part has its own workspace: the parts run concurrently
       grid = std::min(grid, SCRATCH_GRID);
-      if (!s.ptr || s.doubles < (size_t)gws_stride * grid) return -2;
+      if (part < 0 || part >= GQS_PARTS || !s.ptr || s.doubles < (size_t)GQS_PARTS * gws_stride * SCRATCH_GRID) return -2;
+      gws = s.ptr + (size_t)part * gws_stride * SCRATCH_GRID;
    }
    // a type whose workspace leaves room for few workgroups per CU gets wider
    // ones, so that more wavefronts hide the memory latency
    if (total > 32 * 1024)
       hipLaunchKernelGGL(k_gqs<256>, dim3(grid), dim3(256), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
-                         b.tp, t, b, kt, s.ptr, gws_stride, cls_bytes);
+                         b.tp, t, b, kt, gws, gws_stride, cls_bytes, part);
    else
       hipLaunchKernelGGL(k_gqs<128>, dim3(grid), dim3(128), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
-                         b.tp, t, b, kt, s.ptr, gws_stride, cls_bytes);
+                         b.tp, t, b, kt, gws, gws_stride, cls_bytes, part);
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -1,15 +1,24 @@
-// Thread-per-contraction quartet kernel for the small angular-momentum
-// classes, one instantiation per class (la lb|lc ld).
+// Templated quartet kernels for the small angular-momentum classes, one
+// instantiation per class (la lb|lc ld), in two shapes chosen per class by its
+// number of contractions per set quartet.  Both contract each (n1,n2) row's
+// primitive blocks (reference src/compute_ECO.cpp:12-214), run the HRR of the
+// row (d -> c, then b -> a, src/compute_HRR.cpp), sum the rows, transform to
+// spherical (src/compute_SPH.cpp:58-237) and make the four K updates per
+// integral (src/compute_KS.cpp:51-115) with compile-time indices and c2s
+// coefficients, so zero coefficients cost nothing.
 //
-// k_tq<la,lb,lc,ld>: one lane per (set quartet, contraction) of the class.
-// The lane contracts each (n1,n2) row's primitive blocks (reference
-// src/compute_ECO.cpp:12-214), runs the HRR of the row (d -> c, then
-// b -> a, src/compute_HRR.cpp) as straight-line code on registers, sums
-// the rows, transforms to spherical (src/compute_SPH.cpp:58-237) and makes
-// the four K updates per integral (src/compute_KS.cpp:51-115).  Every index
-// and every c2s coefficient is a compile-time constant, so zero coefficients
-// cost nothing and there are no barriers: the lanes of a wavefront that
-// share a set quartet read the same primitive blocks (broadcast loads).
+// k_tql (few contractions): one lane per (set quartet, contraction); the
+// lanes of a set quartet read the same primitive blocks (broadcast loads)
+// and keep the HRR on registers; several set quartets share a wavefront.
+//
+// k_tqw (many contractions): one wavefront per set quartet.  The row's
+// primitive quartets are walked in step on all lanes: lane (c, g) loads
+// column c of each block once (coalesced) and accumulates it into the
+// contractions g, g + LL, g + 2 LL, ... with the coefficient products of the
+// batch, formed once per batch in LDS.  The contracted blocks then go to LDS,
+// one row per contraction (odd compile-time strides, so element offsets are
+// immediates), where each lane runs the HRR, the row sum, the spherical
+// transform and the K updates of its own contractions.
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -167,7 +176,91 @@ __device__ __forceinline__ void sph_stage(const double *in, double *out)
       }
 }
 
-struct TqArgs {
+// one step of HRR on a contraction's LDS row (element i at row[i]): ket
+// layer dd from layer dd-1
+template <int la, int lb, int lc, int ld, int dd>
+__device__ __forceinline__ void ket_step_lds(const double *in, double *out, const double CD[3])
+{
+   using C = Cls<la, lb, lc, ld>;
+#pragma unroll
+   for (int e = la; e <= C::LA; e++)
+#pragma unroll
+      for (int c = lc; c <= C::LC - dd; c++)
+#pragma unroll
+         for (int ie = 0; ie < nco(e); ie++)
+#pragma unroll
+            for (int ic = 0; ic < nco(c); ic++)
+#pragma unroll
+               for (int id = 0; id < nco(dd); id++) {
+                  const int dir = first_dir(dd, id);
+                  const int idm = shift_idx(dd, id, dir, -1);
+                  const int icp = shift_idx(c, ic, dir, 1);
+                  const int i1 = C::koff(dd - 1, e, c + 1) + (ie * nco(c + 1) + icp) * nco(dd - 1) + idm;
+                  const int i0 = C::koff(dd - 1, e, c) + (ie * nco(c) + ic) * nco(dd - 1) + idm;
+                  const int io = C::koff(dd, e, c) + (ie * nco(c) + ic) * nco(dd) + id;
+                  out[io] = in[i1] + CD[dir] * in[i0];
+               }
+}
+
+// bra layer bb from layer bb-1
+template <int la, int lb, int lc, int ld, int bb>
+__device__ __forceinline__ void bra_step_lds(const double *in, double *out, const double AB[3])
+{
+   using C = Cls<la, lb, lc, ld>;
+#pragma unroll
+   for (int a = la; a <= C::LA - bb; a++)
+#pragma unroll
+      for (int ia = 0; ia < nco(a); ia++)
+#pragma unroll
+         for (int ib = 0; ib < nco(bb); ib++)
+#pragma unroll
+            for (int icd = 0; icd < C::ncd; icd++) {
+               const int dir = first_dir(bb, ib);
+               const int ibm = shift_idx(bb, ib, dir, -1);
+               const int iap = shift_idx(a, ia, dir, 1);
+               const int i1 = C::boff(bb - 1, a + 1) + (iap * nco(bb - 1) + ibm) * C::ncd + icd;
+               const int i0 = C::boff(bb - 1, a) + (ia * nco(bb - 1) + ibm) * C::ncd + icd;
+               const int io = C::boff(bb, a) + (ia * nco(bb) + ib) * C::ncd + icd;
+               out[io] = in[i1] + AB[dir] * in[i0];
+            }
+}
+
+// HRR of one contraction of one row: layer 0 in B0, (la lb|lc ld) added to
+// acc; B0 and B1 are both overwritten
+template <int la, int lb, int lc, int ld>
+__device__ __forceinline__ void hrr_lds(double *B0, double *B1, double *acc, const double AB[3], const double CD[3])
+{
+   using C = Cls<la, lb, lc, ld>;
+   double *bufs[2] = {B0, B1};
+   int cur = 0;
+   if constexpr (ld >= 1) { ket_step_lds<la, lb, lc, ld, 1>(bufs[cur], bufs[cur ^ 1], CD); cur ^= 1; }
+   if constexpr (ld >= 2) { ket_step_lds<la, lb, lc, ld, 2>(bufs[cur], bufs[cur ^ 1], CD); cur ^= 1; }
+   if constexpr (ld >= 3) { ket_step_lds<la, lb, lc, ld, 3>(bufs[cur], bufs[cur ^ 1], CD); cur ^= 1; }
+   if constexpr (lb >= 1) { bra_step_lds<la, lb, lc, ld, 1>(bufs[cur], bufs[cur ^ 1], AB); cur ^= 1; }
+   if constexpr (lb >= 2) { bra_step_lds<la, lb, lc, ld, 2>(bufs[cur], bufs[cur ^ 1], AB); cur ^= 1; }
+   if constexpr (lb >= 3) { bra_step_lds<la, lb, lc, ld, 3>(bufs[cur], bufs[cur ^ 1], AB); cur ^= 1; }
+   const double *res = bufs[cur];
+#pragma unroll
+   for (int i = 0; i < C::Nc; i++) acc[i] += res[i];
+}
+
+// one stage of cart -> sph on a contraction's LDS row: in [k][n] -> out [n][m]
+template <int l, int n>
+__device__ __forceinline__ void sph_stage_lds(const double *in, double *out)
+{
+#pragma unroll
+   for (int j = 0; j < n; j++)
+#pragma unroll
+      for (int i = 0; i < 2 * l + 1; i++) {
+         double v = 0.0;
+#pragma unroll
+         for (int k = 0; k < nco(l); k++)
+            if (c2s(l, i, k) != 0.0) v += c2s(l, i, k) * in[k * n + j];
+         out[j * (2 * l + 1) + i] = v;
+      }
+}
+
+struct TqlArgs {
    int nq_all;          // contractions of the class
    int nclass;
    int bsrc[4][4];      // type-block offset of class block (la+e, lc+f)
@@ -176,7 +269,7 @@ struct TqArgs {
 };
 
 template <int la, int lb, int lc, int ld>
-__global__ __launch_bounds__(64) void k_tq(int cls_k, DevTables t, BatchDev b, KTargets kt, TqArgs y)
+__global__ __launch_bounds__(64) void k_tql(int cls_k, DevTables t, BatchDev b, KTargets kt, TqlArgs y)
 {
    using C = Cls<la, lb, lc, ld>;
    constexpr int CAP = C::cap();
@@ -330,10 +423,10 @@ __global__ __launch_bounds__(64) void k_tq(int cls_k, DevTables t, BatchDev b, K
 }
 
 template <int la, int lb, int lc, int ld>
-int launch_one(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt,
-               hipStream_t st)
+int launch_lane(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt,
+                hipStream_t st)
 {
-   TqArgs y;
+   TqlArgs y;
    const int sh = tp.cls_sh[cls_k];
    y.sa = sh & 15;
    y.sb = (sh >> 4) & 15;
@@ -352,8 +445,276 @@ int launch_one(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev
    const size_t grid = (items + 63) / 64;
    if (grid == 0) return 0;
    if (grid > 0x7fffffff) return -4;
-   hipLaunchKernelGGL((k_tq<la, lb, lc, ld>), dim3((unsigned)grid), dim3(64), 0, st, cls_k, t, b, kt, y);
+   hipLaunchKernelGGL((k_tql<la, lb, lc, ld>), dim3((unsigned)grid), dim3(64), 0, st, cls_k, t, b, kt, y);
    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+constexpr int TQ_PB = 8;     // primitive quartets per batch
+constexpr int TQ_LPL = 24;   // contractions per lane in the contraction phase
+constexpr int TQ_NQC = 128;  // contractions per pass, at most
+
+struct TqwArgs {
+   int nclass;
+   int nq_all;          // contractions of the class per set quartet
+   int nqc;             // contractions per pass (<= nq_all)
+   int nla, nlb, nlc, nld;
+   int sa, sb, sc, sd;  // shells of the class on the four centres
+   const int *srcmap;   // class column -> column of the type's blocks
+   int o_kp, o_b0, o_b1, o_acc;   // LDS offsets, doubles (coefficients at 0)
+   int o_code;          // LDS byte offset of the contraction codes ia|ib<<8|ic<<16|id<<24
+};
+
+template <int la, int lb, int lc, int ld>
+struct TqShape {
+   using C = Cls<la, lb, lc, ld>;
+   static_assert(C::ncp <= 64, "one column per lane");
+   static constexpr int LL = 64 / C::ncp;   // lanes per column
+   static constexpr int CAP = C::cap();
+   // LDS row strides of one contraction (odd: the lanes' rows fall in
+   // different banks), compile-time so that element offsets are immediates
+   static constexpr int WB = CAP | 1, WA = C::Nc | 1;
+   // contractions per lane in the contraction phase
+   static constexpr int LPL = (TQ_NQC + LL - 1) / LL < TQ_LPL ? (TQ_NQC + LL - 1) / LL : TQ_LPL;
+};
+
+template <int la, int lb, int lc, int ld>
+__global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, KTargets kt, TqwArgs y)
+{
+   using C = Cls<la, lb, lc, ld>;
+   using S = TqShape<la, lb, lc, ld>;
+   constexpr int NCP = C::ncp, LL = S::LL, NC = C::Nc, NS = C::Ns;
+   extern __shared__ double lds[];
+   const int lane = threadIdx.x;
+   double *co = lds;               // [4 centres][4 contractions][16 primitives]
+   double *kpb = lds + y.o_kp;     // [TQ_PB][nqc]
+   double *B0 = lds + y.o_b0;      // [nqc][WB]
+   double *B1 = lds + y.o_b1;      // [nqc][WB]
+   double *ACC = lds + y.o_acc;    // [nqc][WA]
+   uint32_t *code = (uint32_t *)((char *)lds + y.o_code);   // [nq_all]
+   const int col = lane % NCP, grp = lane / NCP;
+   const bool cact = grp < LL;     // lane takes part in the contraction
+   const int src = cact ? y.srcmap[col] : 0;
+   const int ncpt = b.tp->ncp;
+   const uint32_t nfix = min(b.fix_count[0], b.fix_cap);
+   const int gate = b.tp->cls_gate[cls_k];
+   const int s = y.nqc;
+   {
+      const int nbcd = y.nlb * y.nlc * y.nld, ncd = y.nlc * y.nld;
+      for (int l = lane; l < y.nq_all; l += 64)
+         code[l] = (uint32_t)(l / nbcd) | (uint32_t)((l / ncd) % y.nlb) << 8 | (uint32_t)((l / y.nld) % y.nlc) << 16 |
+                   (uint32_t)(l % y.nld) << 24;
+   }
+   const double symm[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
+
+   for (uint32_t gi = b.g0 + blockIdx.x; gi < b.g1; gi += gridDim.x) {
+      const KsGroup ks = b.ksg[(size_t)gi * y.nclass + cls_k];
+      const int nq = (int)(ks.flags >> 16);
+      if (nq == 0) continue;
+      const GroupRec &g = b.groups[gi];
+      __syncthreads();
+      // the set quartet's coefficients of the class's four shells
+      {
+         const int shv[4] = {y.sa, y.sb, y.sc, y.sd}, nlv[4] = {y.nla, y.nlb, y.nlc, y.nld};
+         for (int i = lane; i < 256; i += 64) {
+            const int X = i >> 6, inl = (i >> 4) & 3, ip = i & 15;
+            const int np = ((g.np4 >> (4 * X)) & 15) + 1;
+            co[i] = (inl < nlv[X] && ip < np) ? b.data[g.Kof[X][shv[X]] + inl * np + ip] : 0.0;
+         }
+      }
+      for (int l0 = 0; l0 < nq; l0 += y.nqc) {
+         const int nqc = min(y.nqc, nq - l0);
+         for (int i = lane; i < S::WA * nqc; i += 64) ACC[i] = 0.0;
+         __syncthreads();
+         for (uint32_t r = g.row0; r < g.row1; r++) {
+            const RowRec row = b.rows[r];
+            double E[S::LPL];
+#pragma unroll
+            for (int k = 0; k < S::LPL; k++) E[k] = 0.0;
+            const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * ncpt + src;
+            const uint32_t *IP = b.prm_ip + row.prm_off;
+            for (uint32_t p0 = 0; p0 < row.n_prm; p0 += TQ_PB) {
+               const int nb = min(TQ_PB, (int)(row.n_prm - p0));
+               double av[TQ_PB];
+#pragma unroll
+               for (int q = 0; q < TQ_PB; q++) av[q] = (cact && q < nb) ? A0[(size_t)(p0 + q) * ncpt] : 0.0;
+               __syncthreads();
+               // coefficient products of the batch, once per (primitive, contraction)
+               for (int q = 0; q < nb; q++) {
+                  const uint32_t ip = IP[p0 + q];
+                  for (int lq = lane; lq < nqc; lq += 64) {
+                     const uint32_t cd = code[l0 + lq];
+                     kpb[q * s + lq] = co[(cd & 255) * 16 + (ip & 15)] * co[64 + ((cd >> 8) & 255) * 16 + ((ip >> 4) & 15)] *
+                                       co[128 + ((cd >> 16) & 255) * 16 + ((ip >> 8) & 15)] *
+                                       co[192 + (cd >> 24) * 16 + ((ip >> 12) & 15)];
+                  }
+               }
+               __syncthreads();
+               if (cact) {
+#pragma unroll
+                  for (int k = 0; k < S::LPL; k++) {
+                     const int lq = grp + k * LL;
+                     if (lq < nqc) {
+#pragma unroll
+                        for (int q = 0; q < TQ_PB; q++)
+                           if (q < nb) E[k] += kpb[q * s + lq] * av[q];
+                     }
+                  }
+               }
+            }
+            // slots of this row whose screening depends on the class (rare)
+            for (uint32_t e2 = 0; e2 < nfix; e2++) {
+               const uint32_t p = b.fix[2 * e2];
+               if (b.prm_row[p] != r) continue;
+               const uint32_t ip = b.prm_ip[p];
+               const double *fsrc = b.fix_ws + (size_t)e2 * (ncpt + 16);
+               const bool keep = ((b.fix[2 * e2 + 1] >> 8) >> gate) & 1u;
+               // a screened slot keeps its Fm[0] in (00|00) only (reference src/compute_VRR.cpp:384-385)
+               const double v = keep ? fsrc[src] : ((la == 0 && lc == 0 && col == 0) ? fsrc[ncpt + gate] : 0.0);
+               if (cact) {
+#pragma unroll
+                  for (int k = 0; k < S::LPL; k++) {
+                     const int lq = grp + k * LL;
+                     if (lq < nqc) {
+                        const uint32_t cd = code[l0 + lq];
+                        E[k] += co[(cd & 255) * 16 + (ip & 15)] * co[64 + ((cd >> 8) & 255) * 16 + ((ip >> 4) & 15)] *
+                                co[128 + ((cd >> 16) & 255) * 16 + ((ip >> 8) & 15)] *
+                                co[192 + (cd >> 24) * 16 + ((ip >> 12) & 15)] * v;
+                     }
+                  }
+               }
+            }
+            // the row's contracted blocks, one LDS row per contraction
+            if (cact) {
+#pragma unroll
+               for (int k = 0; k < S::LPL; k++) {
+                  const int lq = grp + k * LL;
+                  if (lq < nqc) B0[lq * S::WB + col] = E[k];
+               }
+            }
+            __syncthreads();
+            double AB[3], CD[3];
+            {
+               const double *Ao = b.data + g.R[0], *Bo = b.data + g.R[1], *Co = b.data + g.R[2], *Do = b.data + g.R[3];
+               const int n1 = row.n12 & 0xffff, n2 = row.n12 >> 16;
+               double ABs[3], CDs[3];
+               gint_pbc(Ao, Bo, t.cell, ABs);
+               gint_pbc(Co, Do, t.cell, CDs);
+#pragma unroll
+               for (int k = 0; k < 3; k++) {
+                  const double Bk = Ao[k] + ABs[k] + t.neighs[n1 * 3 + k];
+                  const double Dk = Co[k] + CDs[k] + t.neighs[n2 * 3 + k];
+                  AB[k] = Ao[k] - Bk;
+                  CD[k] = Co[k] - Dk;
+               }
+            }
+            for (int lq = lane; lq < nqc; lq += 64)
+               hrr_lds<la, lb, lc, ld>(B0 + lq * S::WB, B1 + lq * S::WB, ACC + lq * S::WA, AB, CD);
+            __syncthreads();
+         }
+         // cartesian -> spherical and the K updates of each contraction
+         constexpr int n0 = nco(lb) * nco(lc) * nco(ld);
+         constexpr int n1 = nco(lc) * nco(ld) * (2 * la + 1);
+         constexpr int n2 = nco(ld) * (2 * la + 1) * (2 * lb + 1);
+         constexpr int n3 = (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1);
+         for (int lq = lane; lq < nqc; lq += 64) {
+            double *b0 = B0 + lq * S::WB, *b1 = B1 + lq * S::WB;
+            sph_stage_lds<la, n0>(ACC + lq * S::WA, b0);
+            sph_stage_lds<lb, n1>(b0, b1);
+            sph_stage_lds<lc, n2>(b1, b0);
+            sph_stage_lds<ld, n3>(b0, b1);
+            const int l = l0 + lq;
+            const uint32_t cd = code[l];
+            const long long ia = cd & 255, ib = (cd >> 8) & 255, ic = (cd >> 16) & 255, id = cd >> 24;
+            size_t off[4];
+            uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
+            if (ks.flags & KS_EXPLICIT) {
+               const KsRec x = b.ksx[ks.expl + l];
+               for (int u = 0; u < 4; u++) off[u] = x.off[u];
+               ld01 = x.ld01;
+               ld23 = x.ld23;
+               flags = x.flags;
+            } else {
+               off[0] = (size_t)((long long)ks.base[0] + ia * ks.step[0] + ic * ks.step[1]);
+               off[1] = (size_t)((long long)ks.base[1] + ia * ks.step[2] + id * ks.step[3]);
+               off[2] = (size_t)((long long)ks.base[2] + ib * ks.step[4] + ic * ks.step[5]);
+               off[3] = (size_t)((long long)ks.base[3] + ib * ks.step[6] + id * ks.step[7]);
+            }
+            const double fac = symm[(flags >> 4) & 7] * kt.hf_fac;
+            if (fac == 0.0) continue;
+            const unsigned ldac = ld01 & 0xffff, ldad = ld01 >> 16, ldbc = ld23 & 0xffff, ldbd = ld23 >> 16;
+            const bool Tac = flags & 1, Tad = flags & 2, Tbc = flags & 4, Tbd = flags & 8;
+            for (int sp = 0; sp < kt.nspin; sp++) {
+               const double *__restrict__ P = kt.P[sp];
+               double *__restrict__ K = kt.K[sp];
+#pragma unroll 4
+               for (int tt = 0; tt < NS; tt++) {
+                  const int isa = tt / ((2 * lb + 1) * (2 * lc + 1) * (2 * ld + 1));
+                  const int isb = (tt / ((2 * lc + 1) * (2 * ld + 1))) % (2 * lb + 1);
+                  const int isc = (tt / (2 * ld + 1)) % (2 * lc + 1), isd = tt % (2 * ld + 1);
+                  const size_t Iac = off[0] + (Tac ? isa * ldac + isc : isc * ldac + isa);
+                  const size_t Iad = off[1] + (Tad ? isa * ldad + isd : isd * ldad + isa);
+                  const size_t Ibc = off[2] + (Tbc ? isb * ldbc + isc : isc * ldbc + isb);
+                  const size_t Ibd = off[3] + (Tbd ? isb * ldbd + isd : isd * ldbd + isb);
+                  const double iabcd = -fac * b1[tt];
+                  const double pac = P[Iac], pad = P[Iad], pbc = P[Ibc], pbd = P[Ibd];
+                  unsafeAtomicAdd(&K[Ibd], iabcd * pac);
+                  unsafeAtomicAdd(&K[Ibc], iabcd * pad);
+                  unsafeAtomicAdd(&K[Iad], iabcd * pbc);
+                  unsafeAtomicAdd(&K[Iac], iabcd * pbd);
+               }
+            }
+         }
+         __syncthreads();
+      }
+   }
+}
+
+template <int la, int lb, int lc, int ld>
+int launch_wave(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt,
+                const int *srcmap, hipStream_t st)
+{
+   using C = Cls<la, lb, lc, ld>;
+   using S = TqShape<la, lb, lc, ld>;
+   TqwArgs y;
+   const int sh = tp.cls_sh[cls_k];
+   y.sa = sh & 15;
+   y.sb = (sh >> 4) & 15;
+   y.sc = (sh >> 8) & 15;
+   y.sd = (sh >> 12) & 15;
+   y.nla = tp.nl[0][y.sa];
+   y.nlb = tp.nl[1][y.sb];
+   y.nlc = tp.nl[2][y.sc];
+   y.nld = tp.nl[3][y.sd];
+   y.nclass = tp.nclass;
+   y.nq_all = tp.cls_nl[cls_k];
+   y.nqc = std::min(std::min(y.nq_all, S::LPL * S::LL), TQ_NQC);
+   y.srcmap = srcmap;
+   const int nqc = y.nqc;
+   y.o_kp = 256;
+   y.o_b0 = y.o_kp + TQ_PB * nqc;
+   y.o_b1 = y.o_b0 + S::WB * nqc;
+   y.o_acc = y.o_b1 + S::WB * nqc;
+   y.o_code = 8 * (y.o_acc + S::WA * nqc);
+   const size_t lds_bytes = (size_t)y.o_code + 4 * (size_t)y.nq_all;
+   if (y.nq_all < 1 || y.nq_all > 256 || lds_bytes > 64 * 1024) return -6;
+   const unsigned ng = b.g1 - b.g0;
+   if (ng == 0) return 0;
+   const unsigned grid = std::min(ng, 1u << 16);
+   hipLaunchKernelGGL((k_tqw<la, lb, lc, ld>), dim3(grid), dim3(64), lds_bytes, st, cls_k, t, b, kt, y);
+   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// set quartets with at least this many contractions of the class get a
+// wavefront each; fewer, and a lane per contraction packs several set
+// quartets into one wavefront
+constexpr int TQ_WAVE_MIN_NQ = 48;
+
+template <int la, int lb, int lc, int ld>
+int launch_one(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt,
+               const int *srcmap, hipStream_t st)
+{
+   if (tp.cls_nl[cls_k] >= TQ_WAVE_MIN_NQ) return launch_wave<la, lb, lc, ld>(cls_k, tp, t, b, kt, srcmap, st);
+   return launch_lane<la, lb, lc, ld>(cls_k, tp, t, b, kt, st);
 }
 
 }  // namespace tq
@@ -378,13 +739,14 @@ bool tq_has(int code)
    }
 }
 
-int launch_tq(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt, void *stream)
+int launch_tq(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt,
+              const int *srcmap, void *stream)
 {
    if (b.g1 <= b.g0) return 0;
    hipStream_t st = (hipStream_t)stream;
    switch (tp.cls_code[cls_k]) {
 #define GINT_TQ_CASE(a, b_, c, d) \
-   case class_code(a, b_, c, d): return tq::launch_one<a, b_, c, d>(cls_k, tp, t, b, kt, st);
+   case class_code(a, b_, c, d): return tq::launch_one<a, b_, c, d>(cls_k, tp, t, b, kt, srcmap, st);
       GINT_TQ_CLASSES(GINT_TQ_CASE)
 #undef GINT_TQ_CASE
       default:
