@@ -327,7 +327,9 @@ static FmLayout fm_layout(const TypePlan &tp, bool fix)
    y.o_fmv = GEO;
    y.o_sv = GEO + 64 * (L + 1);
    y.o_R = y.o_sv + 320;
-   int SB = fix ? 1 : (1792 - y.o_R) / (2 * nh);
+   // the Hermite batch fills what is left of a 10 KB budget: small enough
+   // that 4 wavefronts per SIMD fit in LDS
+   int SB = fix ? 1 : (1280 - y.o_R) / (2 * nh);
    SB = std::max(1, std::min(16, SB));
    y.SB = SB;
    const int cell_phase = y.o_R + 2 * SB * nh;
@@ -338,8 +340,11 @@ static FmLayout fm_layout(const TypePlan &tp, bool fix)
    return y;
 }
 
+// at most 128 VGPRs (4 wavefronts per SIMD, with the LDS budget above): the
+// kernel waits on memory half of its time, so wavefronts in flight pay more
+// than the few values it then keeps in scratch
 template <bool FIX>
-__global__ __launch_bounds__(64) void k_fm_r(const TypePlan *__restrict__ tpp, DevTables t, BatchDev b, FmLayout lay)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_fm_r(const TypePlan *__restrict__ tpp, DevTables t, BatchDev b, FmLayout lay)
 {
    extern __shared__ double lds[];
    const TypePlan &tp = *tpp;
