@@ -814,15 +814,25 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
             }
             __syncthreads();
             const double *A = A0 + (size_t)pc * ncpt;
-            for (int idx = tid; idx < nb * ncp; idx += NT) {
-               const int l = fdiv(idx, ncp), j = idx - l * ncp;
+            // thread (g, j): column j of the step's blocks, loaded once, for the
+            // contractions g, g + nlg, g + 2 nlg, ...
+            const int nlg = max(1, min(NT / ncp, nb));
+            const int per = (nb + nlg - 1) / nlg;
+            for (int t = tid; t < nlg * ncp; t += NT) {
+               const int g = fdiv(t, ncp), j = t - g * ncp;
                const int src = srcmap[j];
-               const double *kp = Kp + l * CQ_PCH;
-               double s = 0.0;
+               double a[CQ_PCH];
 #pragma unroll
-               for (int pp = 0; pp < CQ_PCH; pp++)
-                  if (pp < np) s += kp[pp] * A[(size_t)pp * ncpt + src];
-               WS[l * lws + j] += s;
+               for (int pp = 0; pp < CQ_PCH; pp++) a[pp] = pp < np ? A[(size_t)pp * ncpt + src] : 0.0;
+               for (int k = 0; k < per; k++) {
+                  const int l = g + k * nlg;
+                  if (l >= nb) break;
+                  const double *kp = Kp + l * CQ_PCH;
+                  double s = 0.0;
+#pragma unroll
+                  for (int pp = 0; pp < CQ_PCH; pp++) s += kp[pp] * a[pp];
+                  WS[l * lws + j] += s;
+               }
             }
          }
          // slots of this row whose screening depends on the class
