@@ -26,6 +26,22 @@ PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector and FP64 matrix, dense
 PEAK_HBM_GBS = 8000.0
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload`, from the rocprofv3
+    FETCH_SIZE and WRITE_SIZE passes summarised under profiles/ (a line
+    'hbm_traffic: {json}' per kernel), or (None, None) when none covers them"""
+    import glob
+
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(f) as fh:
+            for line in fh:
+                if line.startswith("hbm_traffic: "):
+                    d = json.loads(line[len("hbm_traffic: "):])
+                    if d.get("kernel") == kernel and d.get("workload") == workload:
+                        return d["bytes_per_launch"], os.path.basename(f)
+    return None, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,8 +140,10 @@ def main():
     # average duration of one launch of the dominant kernel, and its FLOPs per launch
     per_launch_ms = kms[dom] / max(nl[dom], 1)
     achieved = kfl[dom] / (kms[dom] * 1e-3) / 1e12 if kms[dom] > 0 else 0.0
+    traffic, traffic_src = measured_traffic(names[dom], s.describe())
     roof = {"kernel": names[dom], "bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": None,
+            "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": traffic,
+            "traffic_source": traffic_src,
             "launches_per_step": nl[dom], "avg_launch_ms": round(per_launch_ms, 4),
             "kernel_ms": {n: round(m, 3) for n, m in zip(names, kms)},
             "kernel_tflops": {n: round(f / (m * 1e-3) / 1e12, 4) if m > 0 else 0 for n, f, m in zip(names, kfl, kms)}}
@@ -135,7 +153,7 @@ def main():
         import oracle
 
         orc = oracle.load()
-        # a strided sample of the same stream, sized to ~15 s of one core:
+        # a strided sample of the same stream, sized to 10-30 s of one core:
         # a short probe first, then the sample the probe's rate allows
         def run_sample(stride):
             ho, to = D.setup_handle(orc, "oracle_", s, P=P, c0=c0)
@@ -150,7 +168,8 @@ def main():
         so, tc = run_sample(stride)
         if not a.cpu_sample_sets and tc < 5.0:
             sets_per_s = (st.n_sets // stride) / max(tc, 1e-3)
-            stride = max(1, int(st.n_sets / max(1.0, 15.0 * sets_per_s)))
+            # (the probe's set quartets run about twice as fast as the average)
+            stride = max(1, int(st.n_sets / max(1.0, 30.0 * sets_per_s)))
             so, tc = run_sample(stride)
         cpu = {"value": round(so[1] / tc, 1), "unit": "quartets/s", "cores": 1, "kind": "port",
                "sample": f"every {stride}th set quartet of the stream ({st.n_sets // stride} set quartets, "
