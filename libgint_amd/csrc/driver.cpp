@@ -108,6 +108,7 @@ struct GdrvParams {
    double min_oo_A;
    int symmetric;      // 1: unique quartets with weights; 0: every ordered quartet, weight 1/4
    double cell_A;      // cell edge; 0 -> box_A (a larger cell makes the minimum image the identity)
+   int np_tm;          // primitives of the spdf centres (16 in the preset workloads; fewer for tests)
 };
 
 struct GdrvSystem {
@@ -202,7 +203,7 @@ void build(GdrvSystem &S)
       S.kinds.push_back(make_basis(rng, {{0, 3}, {1, 2}, {2, 1}}, rng.range(p.np_min, p.np_max)));
       S.kinds.push_back(make_basis(rng, {{0, 2}, {1, 1}}, rng.range(p.np_min, p.np_max)));
    }
-   S.kinds.push_back(make_basis(rng, {{0, 2}, {1, 2}, {2, 2}, {3, 4}}, 16));
+   S.kinds.push_back(make_basis(rng, {{0, 2}, {1, 2}, {2, 2}, {3, 4}}, p.np_tm > 0 ? p.np_tm : 16));
 
    const double dOH = 0.96 * BOHR_PER_ANG, ang = 104.5 * M_PI / 180.0;
    const double dmin = p.min_oo_A * BOHR_PER_ANG;

@@ -79,6 +79,11 @@ CASES = [
     ("water2_pbc27_fixup", dict(n_water=2, box_A=7.0, image_shells=1, np_max=3, seed=13), dict(debug=1)),
     ("water1_pbc125_coulomb", dict(n_water=1, box_A=6.0, image_shells=2, basis=1, np_max=3, seed=14),
      dict(truncated=False)),
+    # an spdf centre (2s 2p 2d 4f contractions): f-shell classes, four-fold
+    # contracted shells and quartet types up to L = 12, with 2 primitives so
+    # that the oracle finishes in seconds
+    ("water1_tm1_open_spdf", dict(n_water=1, n_tm=1, box_A=5.0, image_shells=0, np_max=2, np_tm=2, seed=3), {}),
+    ("water1_tm1_pbc27_spdf", dict(n_water=1, n_tm=1, box_A=6.0, image_shells=1, np_max=2, np_tm=2, seed=5), {}),
 ]
 
 
