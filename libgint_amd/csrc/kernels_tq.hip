@@ -1,24 +1,22 @@
 // Templated quartet kernels for the small angular-momentum classes, one
-// instantiation per class (la lb|lc ld), in two shapes chosen per class by its
-// number of contractions per set quartet.  Both contract each (n1,n2) row's
+// instantiation per class (la lb|lc ld).  They contract each (n1,n2) row's
 // primitive blocks (reference src/compute_ECO.cpp:12-214), run the HRR of the
 // row (d -> c, then b -> a, src/compute_HRR.cpp), sum the rows, transform to
-// spherical (src/compute_SPH.cpp:58-237) and make the four K updates per
-// integral (src/compute_KS.cpp:51-115) with compile-time indices and c2s
-// coefficients, so zero coefficients cost nothing.
+// spherical (src/compute_SPH.cpp:58-237) and update K (src/compute_KS.cpp:
+// 51-115) with compile-time indices and c2s coefficients, so zero
+// coefficients cost nothing.
 //
-// k_tql (few contractions): one lane per (set quartet, contraction); the
-// lanes of a set quartet read the same primitive blocks (broadcast loads)
-// and keep the HRR on registers; several set quartets share a wavefront.
-//
-// k_tqw (many contractions): one wavefront per set quartet.  The row's
-// primitive quartets are walked in step on all lanes: lane (c, g) loads
-// column c of each block once (coalesced) and accumulates it into the
-// contractions g, g + LL, g + 2 LL, ... with the coefficient products of the
-// batch, formed once per batch in LDS.  The contracted blocks then go to LDS,
-// one row per contraction (odd compile-time strides, so element offsets are
-// immediates), where each lane runs the HRR, the row sum, the spherical
-// transform and the K updates of its own contractions.
+// k_tqw: one wavefront per set quartet.  The row's primitive quartets are
+// walked in step on all lanes: lane (c, g) loads column c of each block once
+// (coalesced) and accumulates it into the contractions g, g + LL, g + 2 LL,
+// ... with the coefficient products of the batch, formed once per batch in
+// LDS.  The contracted blocks then go to LDS, one row per contraction (odd
+// compile-time strides, so element offsets are immediates), where each lane
+// runs the HRR, the row sum and the spherical transform of its own
+// contractions.  The K updates of the set quartet are summed over its
+// contractions before the atomics (one atomic per K element of the four
+// blocks), unless its calls are not affine; then each contraction makes the
+// four updates per integral.
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -95,86 +93,6 @@ struct Cls {
       return c;
    }
 };
-
-// ket HRR step: layer dd (out) from layer dd-1 (in)
-template <int la, int lb, int lc, int ld, int dd>
-__device__ __forceinline__ void ket_step(const double *in, double *out, const double CD[3])
-{
-   using C = Cls<la, lb, lc, ld>;
-#pragma unroll
-   for (int e = la; e <= C::LA; e++)
-#pragma unroll
-      for (int c = lc; c <= C::LC - dd; c++)
-#pragma unroll
-         for (int ie = 0; ie < nco(e); ie++)
-#pragma unroll
-            for (int ic = 0; ic < nco(c); ic++)
-#pragma unroll
-               for (int id = 0; id < nco(dd); id++) {
-                  const int dir = first_dir(dd, id);
-                  const int idm = shift_idx(dd, id, dir, -1);
-                  const int icp = shift_idx(c, ic, dir, 1);
-                  const double v1 = in[C::koff(dd - 1, e, c + 1) + (ie * nco(c + 1) + icp) * nco(dd - 1) + idm];
-                  const double v0 = in[C::koff(dd - 1, e, c) + (ie * nco(c) + ic) * nco(dd - 1) + idm];
-                  out[C::koff(dd, e, c) + (ie * nco(c) + ic) * nco(dd) + id] = v1 + CD[dir] * v0;
-               }
-}
-
-// bra HRR step: layer bb (out) from layer bb-1 (in)
-template <int la, int lb, int lc, int ld, int bb>
-__device__ __forceinline__ void bra_step(const double *in, double *out, const double AB[3])
-{
-   using C = Cls<la, lb, lc, ld>;
-#pragma unroll
-   for (int a = la; a <= C::LA - bb; a++)
-#pragma unroll
-      for (int ia = 0; ia < nco(a); ia++)
-#pragma unroll
-         for (int ib = 0; ib < nco(bb); ib++)
-#pragma unroll
-            for (int icd = 0; icd < C::ncd; icd++) {
-               const int dir = first_dir(bb, ib);
-               const int ibm = shift_idx(bb, ib, dir, -1);
-               const int iap = shift_idx(a, ia, dir, 1);
-               const double v1 = in[C::boff(bb - 1, a + 1) + (iap * nco(bb - 1) + ibm) * C::ncd + icd];
-               const double v0 = in[C::boff(bb - 1, a) + (ia * nco(bb - 1) + ibm) * C::ncd + icd];
-               out[C::boff(bb, a) + (ia * nco(bb) + ib) * C::ncd + icd] = v1 + AB[dir] * v0;
-            }
-}
-
-// HRR of one row: E (layer 0) -> (la lb|lc ld) cartesian added to acc
-template <int la, int lb, int lc, int ld>
-__device__ __forceinline__ void hrr_add(double *E, double *T, double *acc, const double AB[3], const double CD[3])
-{
-   using C = Cls<la, lb, lc, ld>;
-   double *bufs[2] = {E, T};
-   int cur = 0;
-   if constexpr (ld >= 1) { ket_step<la, lb, lc, ld, 1>(bufs[cur], bufs[cur ^ 1], CD); cur ^= 1; }
-   if constexpr (ld >= 2) { ket_step<la, lb, lc, ld, 2>(bufs[cur], bufs[cur ^ 1], CD); cur ^= 1; }
-   if constexpr (ld >= 3) { ket_step<la, lb, lc, ld, 3>(bufs[cur], bufs[cur ^ 1], CD); cur ^= 1; }
-   if constexpr (lb >= 1) { bra_step<la, lb, lc, ld, 1>(bufs[cur], bufs[cur ^ 1], AB); cur ^= 1; }
-   if constexpr (lb >= 2) { bra_step<la, lb, lc, ld, 2>(bufs[cur], bufs[cur ^ 1], AB); cur ^= 1; }
-   if constexpr (lb >= 3) { bra_step<la, lb, lc, ld, 3>(bufs[cur], bufs[cur ^ 1], AB); cur ^= 1; }
-   const double *res = bufs[cur];
-#pragma unroll
-   for (int i = 0; i < C::Nc; i++) acc[i] += res[i];
-}
-
-// one stage of cart -> sph: in [k][n] -> out [n][m] for shell l
-template <int l, int n>
-__device__ __forceinline__ void sph_stage(const double *in, double *out)
-{
-#pragma unroll
-   for (int j = 0; j < n; j++)
-#pragma unroll
-      for (int i = 0; i < 2 * l + 1; i++) {
-         double v = 0.0;
-#pragma unroll
-         for (int q = 0; q < nco(l); q++)
-            if (c2s(l, i, q) != 0.0) v += c2s(l, i, q) * in[q * n + j];
-         out[j * (2 * l + 1) + i] = v;
-      }
-}
 
 // one step of HRR on a contraction's LDS row (element i at row[i]): ket
 // layer dd from layer dd-1
@@ -260,195 +178,6 @@ __device__ __forceinline__ void sph_stage_lds(const double *in, double *out)
       }
 }
 
-struct TqlArgs {
-   int nq_all;          // contractions of the class
-   int nclass;
-   int bsrc[4][4];      // type-block offset of class block (la+e, lc+f)
-   int nla, nlb, nlc, nld;
-   int sa, sb, sc, sd;  // shells of the class on the four centres
-};
-
-template <int la, int lb, int lc, int ld>
-__global__ __launch_bounds__(64) void k_tql(int cls_k, DevTables t, BatchDev b, KTargets kt, TqlArgs y)
-{
-   using C = Cls<la, lb, lc, ld>;
-   constexpr int CAP = C::cap();
-   const uint32_t it = blockIdx.x * 64u + threadIdx.x;
-   const uint32_t gi = b.g0 + it / (uint32_t)y.nq_all;
-   const int l = (int)(it % (uint32_t)y.nq_all);
-   if (gi >= b.g1) return;
-   const KsGroup ks = b.ksg[(size_t)gi * y.nclass + cls_k];
-   const int nq = (int)(ks.flags >> 16);
-   if (l >= nq) return;
-   const GroupRec &g = b.groups[gi];
-   const int ia = l / (y.nlb * y.nlc * y.nld), ib = (l / (y.nlc * y.nld)) % y.nlb;
-   const int ic = (l / y.nld) % y.nlc, id = l % y.nld;
-   const int npa = (g.np4 & 15) + 1, npb = ((g.np4 >> 4) & 15) + 1, npc = ((g.np4 >> 8) & 15) + 1,
-             npd = ((g.np4 >> 12) & 15) + 1;
-   const double *Ka = b.data + g.Kof[0][y.sa] + ia * npa, *Kb = b.data + g.Kof[1][y.sb] + ib * npb;
-   const double *Kc = b.data + g.Kof[2][y.sc] + ic * npc, *Kd = b.data + g.Kof[3][y.sd] + id * npd;
-   const int ncpt = b.tp->ncp;
-   const uint32_t nfix = min(b.fix_count[0], b.fix_cap);
-   const int gate = b.tp->cls_gate[cls_k];
-
-   double acc[C::Nc];
-#pragma unroll
-   for (int i = 0; i < C::Nc; i++) acc[i] = 0.0;
-   for (uint32_t r = g.row0; r < g.row1; r++) {
-      const RowRec row = b.rows[r];
-      double E[CAP], T[CAP];
-#pragma unroll
-      for (int i = 0; i < C::ncp; i++) E[i] = 0.0;
-      const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * ncpt;
-      const uint32_t *IP = b.prm_ip + row.prm_off;
-      // eight primitive quartets at a time: their index loads, then their
-      // coefficient loads, then their blocks, each group issued together
-      constexpr int PB = 8;
-      for (uint32_t p0 = 0; p0 < row.n_prm; p0 += PB) {
-         uint32_t ipv[PB];
-         double kp[PB];
-#pragma unroll
-         for (int q = 0; q < PB; q++) ipv[q] = p0 + q < row.n_prm ? IP[p0 + q] : 0u;
-#pragma unroll
-         for (int q = 0; q < PB; q++) {
-            const uint32_t ip = ipv[q];
-            kp[q] = p0 + q < row.n_prm
-                       ? Ka[ip & 15] * Kb[(ip >> 4) & 15] * Kc[(ip >> 8) & 15] * Kd[(ip >> 12) & 15]
-                       : 0.0;
-         }
-#pragma unroll
-         for (int q = 0; q < PB; q++) {
-            if (p0 + q >= row.n_prm) break;
-            const double *A = A0 + (size_t)(p0 + q) * ncpt;
-#pragma unroll
-            for (int e = la; e <= C::LA; e++)
-#pragma unroll
-               for (int f = lc; f <= C::LC; f++) {
-                  const double *src = A + y.bsrc[e - la][f - lc];
-#pragma unroll
-                  for (int i = 0; i < nco(e) * nco(f); i++) E[C::koff(0, e, f) + i] += kp[q] * src[i];
-               }
-         }
-      }
-      // slots of this row whose screening depends on the class (rare)
-      for (uint32_t e2 = 0; e2 < nfix; e2++) {
-         const uint32_t p = b.fix[2 * e2];
-         if (b.prm_row[p] != r) continue;
-         const uint32_t ip = b.prm_ip[p];
-         const double kp = Ka[ip & 15] * Kb[(ip >> 4) & 15] * Kc[(ip >> 8) & 15] * Kd[(ip >> 12) & 15];
-         const double *src = b.fix_ws + (size_t)e2 * (ncpt + 16);
-         if (((b.fix[2 * e2 + 1] >> 8) >> gate) & 1u) {
-#pragma unroll
-            for (int e = la; e <= C::LA; e++)
-#pragma unroll
-               for (int f = lc; f <= C::LC; f++)
-#pragma unroll
-                  for (int i = 0; i < nco(e) * nco(f); i++)
-                     E[C::koff(0, e, f) + i] += kp * src[y.bsrc[e - la][f - lc] + i];
-         } else if (la == 0 && lc == 0) {
-            // a screened slot keeps its Fm[0] (reference src/compute_VRR.cpp:384-385)
-            E[0] += kp * src[ncpt + gate];
-         }
-      }
-      double AB[3], CD[3];
-      {
-         const double *Ao = b.data + g.R[0], *Bo = b.data + g.R[1], *Co = b.data + g.R[2], *Do = b.data + g.R[3];
-         const int n1 = row.n12 & 0xffff, n2 = row.n12 >> 16;
-         double ABs[3], CDs[3];
-         gint_pbc(Ao, Bo, t.cell, ABs);
-         gint_pbc(Co, Do, t.cell, CDs);
-#pragma unroll
-         for (int k = 0; k < 3; k++) {
-            const double B = Ao[k] + ABs[k] + t.neighs[n1 * 3 + k];
-            const double D = Co[k] + CDs[k] + t.neighs[n2 * 3 + k];
-            AB[k] = Ao[k] - B;
-            CD[k] = Co[k] - D;
-         }
-      }
-      hrr_add<la, lb, lc, ld>(E, T, acc, AB, CD);
-   }
-   // cartesian -> spherical: four stages, each moves the leading index last
-   double s0[C::Nc], s1[C::Nc];
-   constexpr int n0 = nco(lb) * nco(lc) * nco(ld);
-   constexpr int n1 = nco(lc) * nco(ld) * (2 * la + 1);
-   constexpr int n2 = nco(ld) * (2 * la + 1) * (2 * lb + 1);
-   constexpr int n3 = (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1);
-   sph_stage<la, n0>(acc, s0);
-   sph_stage<lb, n1>(s0, s1);
-   sph_stage<lc, n2>(s1, s0);
-   sph_stage<ld, n3>(s0, s1);
-   // exchange update, 4 atomics per spherical integral
-   size_t off[4];
-   uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
-   if (ks.flags & KS_EXPLICIT) {
-      const KsRec x = b.ksx[ks.expl + l];
-      for (int q = 0; q < 4; q++) off[q] = x.off[q];
-      ld01 = x.ld01;
-      ld23 = x.ld23;
-      flags = x.flags;
-   } else {
-      off[0] = (size_t)((long long)ks.base[0] + (long long)ia * ks.step[0] + (long long)ic * ks.step[1]);
-      off[1] = (size_t)((long long)ks.base[1] + (long long)ia * ks.step[2] + (long long)id * ks.step[3]);
-      off[2] = (size_t)((long long)ks.base[2] + (long long)ib * ks.step[4] + (long long)ic * ks.step[5]);
-      off[3] = (size_t)((long long)ks.base[3] + (long long)ib * ks.step[6] + (long long)id * ks.step[7]);
-   }
-   const double symm[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
-   const double fac = symm[(flags >> 4) & 7] * kt.hf_fac;
-   if (fac == 0.0) return;
-   const unsigned ldac = ld01 & 0xffff, ldad = ld01 >> 16, ldbc = ld23 & 0xffff, ldbd = ld23 >> 16;
-   const bool Tac = flags & 1, Tad = flags & 2, Tbc = flags & 4, Tbd = flags & 8;
-#pragma unroll
-   for (int isa = 0; isa < 2 * la + 1; isa++)
-#pragma unroll
-      for (int isb = 0; isb < 2 * lb + 1; isb++)
-#pragma unroll
-         for (int isc = 0; isc < 2 * lc + 1; isc++)
-#pragma unroll
-            for (int isd = 0; isd < 2 * ld + 1; isd++) {
-               const int tt = ((isa * (2 * lb + 1) + isb) * (2 * lc + 1) + isc) * (2 * ld + 1) + isd;
-               const size_t Iac = off[0] + (Tac ? isa * ldac + isc : isc * ldac + isa);
-               const size_t Iad = off[1] + (Tad ? isa * ldad + isd : isd * ldad + isa);
-               const size_t Ibc = off[2] + (Tbc ? isb * ldbc + isc : isc * ldbc + isb);
-               const size_t Ibd = off[3] + (Tbd ? isb * ldbd + isd : isd * ldbd + isb);
-               const double iabcd = -fac * s1[tt];
-               for (int sp = 0; sp < kt.nspin; sp++) {
-                  const double *P = kt.P[sp];
-                  double *K = kt.K[sp];
-                  unsafeAtomicAdd(&K[Ibd], iabcd * P[Iac]);
-                  unsafeAtomicAdd(&K[Ibc], iabcd * P[Iad]);
-                  unsafeAtomicAdd(&K[Iad], iabcd * P[Ibc]);
-                  unsafeAtomicAdd(&K[Iac], iabcd * P[Ibd]);
-               }
-            }
-}
-
-template <int la, int lb, int lc, int ld>
-int launch_lane(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt,
-                hipStream_t st)
-{
-   TqlArgs y;
-   const int sh = tp.cls_sh[cls_k];
-   y.sa = sh & 15;
-   y.sb = (sh >> 4) & 15;
-   y.sc = (sh >> 8) & 15;
-   y.sd = (sh >> 12) & 15;
-   y.nla = tp.nl[0][y.sa];
-   y.nlb = tp.nl[1][y.sb];
-   y.nlc = tp.nl[2][y.sc];
-   y.nld = tp.nl[3][y.sd];
-   y.nq_all = tp.cls_nl[cls_k];
-   y.nclass = tp.nclass;
-   for (int e = 0; e < 4; e++)
-      for (int f = 0; f < 4; f++)
-         y.bsrc[e][f] = (la + e <= la + lb && lc + f <= lc + ld) ? tp.cpoff[la + e][lc + f] : 0;
-   const size_t items = (size_t)(b.g1 - b.g0) * (size_t)y.nq_all;
-   const size_t grid = (items + 63) / 64;
-   if (grid == 0) return 0;
-   if (grid > 0x7fffffff) return -4;
-   hipLaunchKernelGGL((k_tql<la, lb, lc, ld>), dim3((unsigned)grid), dim3(64), 0, st, cls_k, t, b, kt, y);
-   return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
 constexpr int TQ_PB = 8;     // primitive quartets per batch
 constexpr int TQ_LPL = 24;   // contractions per lane in the contraction phase
 constexpr int TQ_NQC = 128;  // contractions per pass, at most
@@ -464,6 +193,91 @@ struct TqwArgs {
    int o_code;          // LDS byte offset of the contraction codes ia|ib<<8|ic<<16|id<<24
 };
 
+// The K updates of one (set quartet, class), summed over the set quartet's
+// contractions before the atomics (the same sums as one update per integral,
+// reference src/compute_KS.cpp:51-115): K_bd += -f sum_ac I P_ac, K_bc +=
+// -f sum_ad I P_ad, K_ad += -f sum_bc I P_bc, K_ac += -f sum_bd I P_bd.
+// I of contraction q, integral t is at I[q * WB + t]; one atomic per K
+// element of the four blocks.
+template <int la, int lb, int lc, int ld, int WB, typename Y>
+__device__ void ks_reduced_impl(const KsGroup &ks, const Y &y, const double *I, const KTargets &kt, int lane)
+{
+   constexpr int NSA = 2 * la + 1, NSB = 2 * lb + 1, NSC = 2 * lc + 1, NSD = 2 * ld + 1;
+   const double symm[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
+   const double fac = symm[(ks.flags >> 4) & 7] * kt.hf_fac;
+   if (fac == 0.0) return;
+   const int nla = y.nla, nlb = y.nlb, nlc = y.nlc, nld = y.nld;
+   const unsigned ldac = ks.ld01 & 0xffff, ldad = ks.ld01 >> 16, ldbc = ks.ld23 & 0xffff, ldbd = ks.ld23 >> 16;
+   const bool Tac = ks.flags & 1, Tad = ks.flags & 2, Tbc = ks.flags & 4, Tbd = ks.flags & 8;
+   auto blk = [](bool T, unsigned ldx, int r, int c) { return T ? (size_t)r * ldx + c : (size_t)c * ldx + r; };
+   auto q_of = [&](int ia, int ib, int ic, int id) { return ((ia * nlb + ib) * nlc + ic) * nld + id; };
+   auto t_of = [](int isa, int isb, int isc, int isd) { return ((isa * NSB + isb) * NSC + isc) * NSD + isd; };
+   for (int sp = 0; sp < kt.nspin; sp++) {
+      const double *__restrict__ P = kt.P[sp];
+      double *__restrict__ K = kt.K[sp];
+      for (int x = lane; x < nlb * NSB * nld * NSD; x += 64) {   // K_bd
+         const int ib = x / (NSB * nld * NSD), isb = (x / (nld * NSD)) % NSB, id = (x / NSD) % nld, isd = x % NSD;
+         double sum = 0.0;
+         for (int ia = 0; ia < nla; ia++)
+            for (int ic = 0; ic < nlc; ic++) {
+               const size_t o = (size_t)((long long)ks.base[0] + (long long)ia * ks.step[0] + (long long)ic * ks.step[1]);
+               const double *Iq = I + q_of(ia, ib, ic, id) * WB;
+#pragma unroll
+               for (int isa = 0; isa < NSA; isa++)
+#pragma unroll
+                  for (int isc = 0; isc < NSC; isc++) sum += Iq[t_of(isa, isb, isc, isd)] * P[o + blk(Tac, ldac, isa, isc)];
+            }
+         const size_t ob = (size_t)((long long)ks.base[3] + (long long)ib * ks.step[6] + (long long)id * ks.step[7]);
+         unsafeAtomicAdd(&K[ob + blk(Tbd, ldbd, isb, isd)], -fac * sum);
+      }
+      for (int x = lane; x < nlb * NSB * nlc * NSC; x += 64) {   // K_bc
+         const int ib = x / (NSB * nlc * NSC), isb = (x / (nlc * NSC)) % NSB, ic = (x / NSC) % nlc, isc = x % NSC;
+         double sum = 0.0;
+         for (int ia = 0; ia < nla; ia++)
+            for (int id = 0; id < nld; id++) {
+               const size_t o = (size_t)((long long)ks.base[1] + (long long)ia * ks.step[2] + (long long)id * ks.step[3]);
+               const double *Iq = I + q_of(ia, ib, ic, id) * WB;
+#pragma unroll
+               for (int isa = 0; isa < NSA; isa++)
+#pragma unroll
+                  for (int isd = 0; isd < NSD; isd++) sum += Iq[t_of(isa, isb, isc, isd)] * P[o + blk(Tad, ldad, isa, isd)];
+            }
+         const size_t ob = (size_t)((long long)ks.base[2] + (long long)ib * ks.step[4] + (long long)ic * ks.step[5]);
+         unsafeAtomicAdd(&K[ob + blk(Tbc, ldbc, isb, isc)], -fac * sum);
+      }
+      for (int x = lane; x < nla * NSA * nld * NSD; x += 64) {   // K_ad
+         const int ia = x / (NSA * nld * NSD), isa = (x / (nld * NSD)) % NSA, id = (x / NSD) % nld, isd = x % NSD;
+         double sum = 0.0;
+         for (int ib = 0; ib < nlb; ib++)
+            for (int ic = 0; ic < nlc; ic++) {
+               const size_t o = (size_t)((long long)ks.base[2] + (long long)ib * ks.step[4] + (long long)ic * ks.step[5]);
+               const double *Iq = I + q_of(ia, ib, ic, id) * WB;
+#pragma unroll
+               for (int isb = 0; isb < NSB; isb++)
+#pragma unroll
+                  for (int isc = 0; isc < NSC; isc++) sum += Iq[t_of(isa, isb, isc, isd)] * P[o + blk(Tbc, ldbc, isb, isc)];
+            }
+         const size_t ob = (size_t)((long long)ks.base[1] + (long long)ia * ks.step[2] + (long long)id * ks.step[3]);
+         unsafeAtomicAdd(&K[ob + blk(Tad, ldad, isa, isd)], -fac * sum);
+      }
+      for (int x = lane; x < nla * NSA * nlc * NSC; x += 64) {   // K_ac
+         const int ia = x / (NSA * nlc * NSC), isa = (x / (nlc * NSC)) % NSA, ic = (x / NSC) % nlc, isc = x % NSC;
+         double sum = 0.0;
+         for (int ib = 0; ib < nlb; ib++)
+            for (int id = 0; id < nld; id++) {
+               const size_t o = (size_t)((long long)ks.base[3] + (long long)ib * ks.step[6] + (long long)id * ks.step[7]);
+               const double *Iq = I + q_of(ia, ib, ic, id) * WB;
+#pragma unroll
+               for (int isb = 0; isb < NSB; isb++)
+#pragma unroll
+                  for (int isd = 0; isd < NSD; isd++) sum += Iq[t_of(isa, isb, isc, isd)] * P[o + blk(Tbd, ldbd, isb, isd)];
+            }
+         const size_t ob = (size_t)((long long)ks.base[0] + (long long)ia * ks.step[0] + (long long)ic * ks.step[1]);
+         unsafeAtomicAdd(&K[ob + blk(Tac, ldac, isa, isc)], -fac * sum);
+      }
+   }
+}
+
 template <int la, int lb, int lc, int ld>
 struct TqShape {
    using C = Cls<la, lb, lc, ld>;
@@ -476,6 +290,12 @@ struct TqShape {
    // contractions per lane in the contraction phase
    static constexpr int LPL = (TQ_NQC + LL - 1) / LL < TQ_LPL ? (TQ_NQC + LL - 1) / LL : TQ_LPL;
 };
+
+template <int la, int lb, int lc, int ld, typename Y>
+__device__ __forceinline__ void ks_reduced(const KsGroup &ks, const Y &y, const double *I, const KTargets &kt, int lane)
+{
+   ks_reduced_impl<la, lb, lc, ld, TqShape<la, lb, lc, ld>::WB>(ks, y, I, kt, lane);
+}
 
 template <int la, int lb, int lc, int ld>
 __global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, KTargets kt, TqwArgs y)
@@ -616,12 +436,16 @@ __global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, 
          constexpr int n1 = nco(lc) * nco(ld) * (2 * la + 1);
          constexpr int n2 = nco(ld) * (2 * la + 1) * (2 * lb + 1);
          constexpr int n3 = (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1);
+         // all of the set quartet's contractions in one pass, with affine
+         // offsets: the K updates are summed over its contractions first
+         const bool reduce = !(ks.flags & KS_EXPLICIT) && nq == y.nq_all && nqc == nq;
          for (int lq = lane; lq < nqc; lq += 64) {
             double *b0 = B0 + lq * S::WB, *b1 = B1 + lq * S::WB;
             sph_stage_lds<la, n0>(ACC + lq * S::WA, b0);
             sph_stage_lds<lb, n1>(b0, b1);
             sph_stage_lds<lc, n2>(b1, b0);
             sph_stage_lds<ld, n3>(b0, b1);
+            if (reduce) continue;
             const int l = l0 + lq;
             const uint32_t cd = code[l];
             const long long ia = cd & 255, ib = (cd >> 8) & 255, ic = (cd >> 16) & 255, id = cd >> 24;
@@ -664,6 +488,10 @@ __global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, 
                }
             }
          }
+         if (reduce) {
+            __syncthreads();
+            ks_reduced<la, lb, lc, ld>(ks, y, B1, kt, lane);
+         }
          __syncthreads();
       }
    }
@@ -704,17 +532,11 @@ int launch_wave(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDe
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// set quartets with at least this many contractions of the class get a
-// wavefront each; fewer, and a lane per contraction packs several set
-// quartets into one wavefront
-constexpr int TQ_WAVE_MIN_NQ = 48;
-
 template <int la, int lb, int lc, int ld>
 int launch_one(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt,
                const int *srcmap, hipStream_t st)
 {
-   if (tp.cls_nl[cls_k] >= TQ_WAVE_MIN_NQ) return launch_wave<la, lb, lc, ld>(cls_k, tp, t, b, kt, srcmap, st);
-   return launch_lane<la, lb, lc, ld>(cls_k, tp, t, b, kt, st);
+   return launch_wave<la, lb, lc, ld>(cls_k, tp, t, b, kt, srcmap, st);
 }
 
 }  // namespace tq
