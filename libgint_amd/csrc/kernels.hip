@@ -709,6 +709,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // ---------------------------------------------------------------------------
 constexpr int CPL_BYTES = (sizeof(ClassPlan) + 15) & ~15;   // LDS copy of the class plan
 constexpr int CQ_PCH = 8;    // primitive quartets per contraction step
+// LDS copies of the tables that the HRR and the spherical transform index per
+// lane (from constant memory they would be vector loads): c2s (110 doubles),
+// then the cartesian first direction [4][10], minus [4][10][3] and plus
+// [7][28][3] tables
+constexpr int CQ_TAB_C2S = 0, CQ_TAB_FIRST = 8 * GINT_C2S_SIZE, CQ_TAB_MINUS = CQ_TAB_FIRST + 40,
+              CQ_TAB_PLUS = CQ_TAB_MINUS + 120, CQ_TAB_BYTES = (CQ_TAB_PLUS + 588 + 15) & ~15;
 
 
 __device__ __forceinline__ double *lds_base()
@@ -762,8 +768,11 @@ __device__ __forceinline__ unsigned blk_idx(bool T, unsigned ld, unsigned row_i,
 template <int NT>
 __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b,
                        const KTargets &kt, const CqLayout &y, double *WS, const int *__restrict__ srcmap, uint32_t gi,
-                       const double *co, uint32_t *ipb)
+                       const double *co, uint32_t *ipb, const char *tabs)
 {
+   const unsigned char *t_first = (const unsigned char *)(tabs + CQ_TAB_FIRST);
+   const signed char *t_minus = (const signed char *)(tabs + CQ_TAB_MINUS);
+   const signed char *t_plus = (const signed char *)(tabs + CQ_TAB_PLUS);
    extern __shared__ double lds[];
    const int tid = threadIdx.x;
    const int cap = cp.ycap, lws = y.lws;
@@ -883,9 +892,9 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
                const int ie = idx / (ncc * nd);
                const int rem = idx - ie * ncc * nd;
                const int ic = rem / nd, id = rem - ic * nd;
-               const int dir = c_cart.first[dd][id];
-               const int idm = c_cart.minus[dd][id][dir];
-               const int icp = c_cart.plus[c][ic][dir];
+               const int dir = t_first[dd * 10 + id];
+               const int idm = t_minus[(dd * 10 + id) * 3 + dir];
+               const int icp = t_plus[(c * 28 + ic) * 3 + dir];
                const double v1 = in[cp.koff[dd - 1][e][c + 1] + (ie * ncc1 + icp) * ndm + idm];
                const double v0 = in[cp.koff[dd - 1][e][c] + (ie * ncc + ic) * ndm + idm];
                WS[l * lws + dst * cap + idx0] = v1 + CD[dir] * v0;
@@ -907,9 +916,9 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
                const int ia = idx / (nbb * ncd);
                const int rem = idx - ia * nbb * ncd;
                const int ib = rem / ncd, icd = rem - ib * ncd;
-               const int dir = c_cart.first[bb][ib];
-               const int ibm = c_cart.minus[bb][ib][dir];
-               const int iap = c_cart.plus[a][ia][dir];
+               const int dir = t_first[bb * 10 + ib];
+               const int ibm = t_minus[(bb * 10 + ib) * 3 + dir];
+               const int iap = t_plus[(a * 28 + ia) * 3 + dir];
                const double v1 = in[cp.boff[bb - 1][a + 1] + (iap * nbm + ibm) * ncd + icd];
                const double v0 = in[cp.boff[bb - 1][a] + (ia * nbm + ibm) * ncd + icd];
                WS[l * lws + dst * cap + idx0] = v1 + AB[dir] * v0;
@@ -934,7 +943,7 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
             const int k = dims[0];
             const int n = dims[1] * dims[2] * dims[3];
             const int m = 2 * ls[s] + 1;
-            const double *c2s = c_c2s + c_c2s_off[ls[s]];
+            const double *c2s = (const double *)(tabs + CQ_TAB_C2S) + c_c2s_off[ls[s]];
             for (int idx1 = tid; idx1 < nb * n * m; idx1 += NT) {
                const int l = fdiv(idx1, n * m), idx = idx1 - l * n * m;
                const int j = idx / m, i = idx - j * m;
@@ -1078,10 +1087,17 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 {
    const TypePlan &tp = *tpp;
    const int tid = threadIdx.x;
-   // LDS: [class workspace, cls_bytes][class plan][coefficients nco_tot][primitive indices CQ_PCH]
+   // LDS: [class workspace, cls_bytes][class plan][coefficients nco_tot][primitive indices CQ_PCH][tables]
    ClassPlan *cpl = (ClassPlan *)((char *)lds_base() + cls_bytes);
    double *co = (double *)((char *)cpl + CPL_BYTES);
    uint32_t *ipb = (uint32_t *)(co + tp.nco_tot);
+   char *tabs = (char *)lds_base() + ((cls_bytes + CPL_BYTES + 8 * tp.nco_tot + 4 * CQ_PCH + 15) & ~15);
+   for (int i = tid; i < GINT_C2S_SIZE; i += NT) ((double *)(tabs + CQ_TAB_C2S))[i] = c_c2s[i];
+   for (int i = tid; i < 40; i += NT) ((unsigned char *)(tabs + CQ_TAB_FIRST))[i] = c_cart.first[i / 10][i % 10];
+   for (int i = tid; i < 120; i += NT)
+      ((signed char *)(tabs + CQ_TAB_MINUS))[i] = c_cart.minus[i / 30][(i / 3) % 10][i % 3];
+   for (int i = tid; i < 588; i += NT) ((signed char *)(tabs + CQ_TAB_PLUS))[i] = c_cart.plus[i / 84][(i / 3) % 28][i % 3];
+   __syncthreads();
    for (uint32_t gi = b.g0 + blockIdx.x; gi < b.g1; gi += gridDim.x) {
       const KsGroup *ksrow = b.ksg + (size_t)gi * tp.nclass;
       const GroupRec &g = b.groups[gi];
@@ -1107,7 +1123,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
          }
          __syncthreads();
          double *WS = y.ws_global ? gws + (size_t)blockIdx.x * gws_stride : lds_base() + y.o_ws;
-         cq_one<NT>(*cpl, k, tp, t, b, kt, y, WS, srcmaps + y.o_src, gi, co, ipb);
+         cq_one<NT>(*cpl, k, tp, t, b, kt, y, WS, srcmaps + y.o_src, gi, co, ipb, tabs);
       }
       __syncthreads();
    }
@@ -1180,7 +1196,7 @@ void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::ve
    for (int k = 0; k < tp.nclass; k++) {
       if (tq_has(tp.cls_code[k])) continue;
       const int need = 3 * cps[k].ycap + cps[k].Nc + CQ_PCH + tp.cls_nl[k] * cps[k].Ns + CQ_PCH;
-      budget = std::max(budget, std::min(need, 8192 - 160 - tp.nco_tot - 4));
+      budget = std::max(budget, std::min(need, 8192 - 160 - tp.nco_tot - 4 - CQ_TAB_BYTES / 8 - 2));
    }
    lays.clear();
    srcmaps.clear();
@@ -1225,7 +1241,7 @@ int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lay
 {
    if (b.g1 <= b.g0) return 0;
    const int cls_bytes = (lds_bytes + 15) & ~15;
-   const size_t total = (size_t)cls_bytes + CPL_BYTES + 8 * (size_t)tp.nco_tot + 4 * CQ_PCH;
+   const size_t total = (size_t)cls_bytes + CPL_BYTES + 8 * (size_t)tp.nco_tot + 4 * CQ_PCH + 16 + CQ_TAB_BYTES;
    if (total > LDS_LIMIT) return -3;
    const uint32_t ng = b.g1 - b.g0;
    unsigned grid = std::min(ng, 1u << 20);
