@@ -1,5 +1,5 @@
-// Templated quartet kernels for the small angular-momentum classes, one
-// instantiation per class (la lb|lc ld).  They contract each (n1,n2) row's
+// Templated quartet kernels for the classes whose (e0|f0) blocks have at most
+// 64 columns, one instantiation per class (la lb|lc ld).  They contract each (n1,n2) row's
 // primitive blocks (reference src/compute_ECO.cpp:12-214), run the HRR of the
 // row (d -> c, then b -> a, src/compute_HRR.cpp), sum the rows, transform to
 // spherical (src/compute_SPH.cpp:58-237) and update K (src/compute_KS.cpp:
@@ -547,7 +547,9 @@ int launch_one(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev
    X(1, 1, 0, 0) X(1, 0, 1, 0) X(0, 0, 1, 1) X(0, 2, 0, 0) X(0, 0, 0, 2) X(1, 0, 0, 1) X(0, 1, 1, 0)          \
    X(0, 1, 0, 1) X(2, 1, 0, 0) X(0, 0, 2, 1) X(2, 0, 1, 0) X(1, 0, 2, 0) X(1, 2, 0, 0) X(0, 0, 1, 2)          \
    X(2, 0, 0, 1) X(0, 1, 2, 0) X(1, 1, 1, 0) X(1, 0, 1, 1) X(1, 0, 0, 2) X(0, 2, 1, 0) X(0, 2, 0, 1)          \
-   X(1, 1, 0, 1) X(0, 1, 1, 1) X(0, 1, 0, 2) X(2, 0, 2, 0) X(2, 2, 0, 0) X(0, 0, 2, 2)
+   X(1, 1, 0, 1) X(0, 1, 1, 1) X(0, 1, 0, 2) X(2, 0, 2, 0) X(2, 2, 0, 0) X(0, 0, 2, 2) X(1, 0, 2, 1)          \
+   X(1, 0, 1, 2) X(0, 1, 2, 1) X(2, 0, 0, 2) X(2, 0, 1, 1) X(0, 2, 2, 0) X(1, 1, 2, 0) X(2, 1, 1, 0)          \
+   X(2, 1, 0, 1) X(1, 2, 1, 0)
 
 bool tq_has(int code)
 {
