@@ -1,5 +1,5 @@
 // Templated quartet kernels for the classes whose (e0|f0) blocks have at most
-// 64 columns, one instantiation per class (la lb|lc ld).  They contract each (n1,n2) row's
+// 128 columns, one instantiation per class (la lb|lc ld).  They contract each (n1,n2) row's
 // primitive blocks (reference src/compute_ECO.cpp:12-214), run the HRR of the
 // row (d -> c, then b -> a, src/compute_HRR.cpp), sum the rows, transform to
 // spherical (src/compute_SPH.cpp:58-237) and update K (src/compute_KS.cpp:
@@ -8,7 +8,8 @@
 //
 // k_tqw: one wavefront per set quartet.  The row's primitive quartets are
 // walked in step on all lanes: lane (c, g) loads column c of each block once
-// (coalesced) and accumulates it into the contractions g, g + LL, g + 2 LL,
+// (coalesced; columns c and c + 64 above 64 columns) and accumulates it into
+// the contractions g, g + LL, g + 2 LL,
 // ... with the coefficient products of the batch, formed once per batch in
 // LDS.  The contracted blocks then go to LDS, one row per contraction (odd
 // compile-time strides, so element offsets are immediates), where each lane
@@ -281,14 +282,15 @@ __device__ void ks_reduced_impl(const KsGroup &ks, const Y &y, const double *I, 
 template <int la, int lb, int lc, int ld>
 struct TqShape {
    using C = Cls<la, lb, lc, ld>;
-   static_assert(C::ncp <= 64, "one column per lane");
-   static constexpr int LL = 64 / C::ncp;   // lanes per column
+   static_assert(C::ncp <= 128, "at most two columns per lane");
+   static constexpr int NCOL = C::ncp > 64 ? 2 : 1;            // columns per lane
+   static constexpr int LL = NCOL == 1 ? 64 / C::ncp : 1;     // lanes per column
    static constexpr int CAP = C::cap();
    // LDS row strides of one contraction (odd: the lanes' rows fall in
    // different banks), compile-time so that element offsets are immediates
    static constexpr int WB = CAP | 1, WA = C::Nc | 1;
    // contractions per lane in the contraction phase
-   static constexpr int LPL = (TQ_NQC + LL - 1) / LL < TQ_LPL ? (TQ_NQC + LL - 1) / LL : TQ_LPL;
+   static constexpr int LPL = NCOL == 2 ? 12 : ((TQ_NQC + LL - 1) / LL < TQ_LPL ? (TQ_NQC + LL - 1) / LL : TQ_LPL);
 };
 
 template <int la, int lb, int lc, int ld, typename Y>
@@ -311,9 +313,19 @@ __global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, 
    double *B1 = lds + y.o_b1;      // [nqc][WB]
    double *ACC = lds + y.o_acc;    // [nqc][WA]
    uint32_t *code = (uint32_t *)((char *)lds + y.o_code);   // [nq_all]
-   const int col = lane % NCP, grp = lane / NCP;
-   const bool cact = grp < LL;     // lane takes part in the contraction
-   const int src = cact ? y.srcmap[col] : 0;
+   // the lane's columns (lane, and lane + 64 when the class has more than 64)
+   // and its group of contractions
+   constexpr int NCOL = S::NCOL;
+   const int grp = NCOL == 1 ? lane / NCP : 0;
+   int colv[NCOL], srcv[NCOL];
+   bool cv[NCOL];
+#pragma unroll
+   for (int c = 0; c < NCOL; c++) {
+      colv[c] = NCOL == 1 ? lane % NCP : lane + 64 * c;
+      cv[c] = NCOL == 1 ? grp < LL : colv[c] < NCP;
+      srcv[c] = cv[c] ? y.srcmap[colv[c]] : 0;
+   }
+   const bool cact = cv[0];        // lane takes part in the contraction
    const int ncpt = b.tp->ncp;
    const uint32_t nfix = min(b.fix_count[0], b.fix_cap);
    const int gate = b.tp->cls_gate[cls_k];
@@ -347,16 +359,20 @@ __global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, 
          __syncthreads();
          for (uint32_t r = g.row0; r < g.row1; r++) {
             const RowRec row = b.rows[r];
-            double E[S::LPL];
+            double E[NCOL][S::LPL];
 #pragma unroll
-            for (int k = 0; k < S::LPL; k++) E[k] = 0.0;
-            const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * ncpt + src;
+            for (int c = 0; c < NCOL; c++)
+#pragma unroll
+               for (int k = 0; k < S::LPL; k++) E[c][k] = 0.0;
+            const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * ncpt;
             const uint32_t *IP = b.prm_ip + row.prm_off;
             for (uint32_t p0 = 0; p0 < row.n_prm; p0 += TQ_PB) {
                const int nb = min(TQ_PB, (int)(row.n_prm - p0));
-               double av[TQ_PB];
+               double av[NCOL][TQ_PB];
 #pragma unroll
-               for (int q = 0; q < TQ_PB; q++) av[q] = (cact && q < nb) ? A0[(size_t)(p0 + q) * ncpt] : 0.0;
+               for (int c = 0; c < NCOL; c++)
+#pragma unroll
+                  for (int q = 0; q < TQ_PB; q++) av[c][q] = (cv[c] && q < nb) ? A0[(size_t)(p0 + q) * ncpt + srcv[c]] : 0.0;
                __syncthreads();
                // coefficient products of the batch, once per (primitive, contraction)
                for (int q = 0; q < nb; q++) {
@@ -376,7 +392,11 @@ __global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, 
                      if (lq < nqc) {
 #pragma unroll
                         for (int q = 0; q < TQ_PB; q++)
-                           if (q < nb) E[k] += kpb[q * s + lq] * av[q];
+                           if (q < nb) {
+                              const double kq = kpb[q * s + lq];
+#pragma unroll
+                              for (int c = 0; c < NCOL; c++) E[c][k] += kq * av[c][q];
+                           }
                      }
                   }
                }
@@ -389,16 +409,22 @@ __global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, 
                const double *fsrc = b.fix_ws + (size_t)e2 * (ncpt + 16);
                const bool keep = ((b.fix[2 * e2 + 1] >> 8) >> gate) & 1u;
                // a screened slot keeps its Fm[0] in (00|00) only (reference src/compute_VRR.cpp:384-385)
-               const double v = keep ? fsrc[src] : ((la == 0 && lc == 0 && col == 0) ? fsrc[ncpt + gate] : 0.0);
+               double v[NCOL];
+#pragma unroll
+               for (int c = 0; c < NCOL; c++)
+                  v[c] = !cv[c] ? 0.0
+                                : (keep ? fsrc[srcv[c]] : ((la == 0 && lc == 0 && colv[c] == 0) ? fsrc[ncpt + gate] : 0.0));
                if (cact) {
 #pragma unroll
                   for (int k = 0; k < S::LPL; k++) {
                      const int lq = grp + k * LL;
                      if (lq < nqc) {
                         const uint32_t cd = code[l0 + lq];
-                        E[k] += co[(cd & 255) * 16 + (ip & 15)] * co[64 + ((cd >> 8) & 255) * 16 + ((ip >> 4) & 15)] *
-                                co[128 + ((cd >> 16) & 255) * 16 + ((ip >> 8) & 15)] *
-                                co[192 + (cd >> 24) * 16 + ((ip >> 12) & 15)] * v;
+                        const double kq = co[(cd & 255) * 16 + (ip & 15)] * co[64 + ((cd >> 8) & 255) * 16 + ((ip >> 4) & 15)] *
+                                          co[128 + ((cd >> 16) & 255) * 16 + ((ip >> 8) & 15)] *
+                                          co[192 + (cd >> 24) * 16 + ((ip >> 12) & 15)];
+#pragma unroll
+                        for (int c = 0; c < NCOL; c++) E[c][k] += kq * v[c];
                      }
                   }
                }
@@ -408,7 +434,11 @@ __global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, 
 #pragma unroll
                for (int k = 0; k < S::LPL; k++) {
                   const int lq = grp + k * LL;
-                  if (lq < nqc) B0[lq * S::WB + col] = E[k];
+                  if (lq < nqc) {
+#pragma unroll
+                     for (int c = 0; c < NCOL; c++)
+                        if (cv[c]) B0[lq * S::WB + colv[c]] = E[c][k];
+                  }
                }
             }
             __syncthreads();
@@ -549,7 +579,9 @@ int launch_one(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev
    X(2, 0, 0, 1) X(0, 1, 2, 0) X(1, 1, 1, 0) X(1, 0, 1, 1) X(1, 0, 0, 2) X(0, 2, 1, 0) X(0, 2, 0, 1)          \
    X(1, 1, 0, 1) X(0, 1, 1, 1) X(0, 1, 0, 2) X(2, 0, 2, 0) X(2, 2, 0, 0) X(0, 0, 2, 2) X(1, 0, 2, 1)          \
    X(1, 0, 1, 2) X(0, 1, 2, 1) X(2, 0, 0, 2) X(2, 0, 1, 1) X(0, 2, 2, 0) X(1, 1, 2, 0) X(2, 1, 1, 0)          \
-   X(2, 1, 0, 1) X(1, 2, 1, 0)
+   X(2, 1, 0, 1) X(1, 2, 1, 0) X(1, 1, 1, 1) X(1, 0, 2, 2) X(2, 2, 1, 0) X(0, 2, 1, 1) X(1, 1, 0, 2)          \
+   X(2, 1, 2, 0) X(2, 0, 2, 1) X(1, 2, 0, 1) X(0, 1, 1, 2) X(1, 2, 2, 0) X(2, 0, 1, 2) X(2, 2, 0, 1)          \
+   X(0, 1, 2, 2) X(0, 2, 0, 2)
 
 bool tq_has(int code)
 {
