@@ -320,12 +320,13 @@ struct FmLayout {
 constexpr int GEO = 24;   // doubles of the primitive geometry at the start of LDS
 enum { G_P = 0, G_Q = 3, G_PA = 6, G_QC = 9, G_KFAC = 12, G_RHO = 13, G_RR = 14, G_I2AB = 15, G_I2CD = 16 };
 
-static FmLayout fm_layout(const TypePlan &tp, bool fix)
+static FmLayout fm_layout(const TypePlan &tp, bool fix, bool pair = false)
 {
    FmLayout y;
    const int L = tp.L, nh = tp.nh;
-   y.o_fmv = GEO;
-   y.o_sv = GEO + 64 * (L + 1);
+   const int geo = pair ? 2 * GEO : GEO;   // one geometry record per primitive quartet of the wavefront
+   y.o_fmv = geo;
+   y.o_sv = geo + 64 * (L + 1);
    y.o_R = y.o_sv + 320;
    // the Hermite batch fills what is left of a 10 KB budget: small enough
    // that 4 wavefronts per SIMD fit in LDS
@@ -333,7 +334,7 @@ static FmLayout fm_layout(const TypePlan &tp, bool fix)
    SB = std::max(1, std::min(16, SB));
    y.SB = SB;
    const int cell_phase = y.o_R + 2 * SB * nh;
-   const int e_phase = GEO + nh + 6 * 28 + tp.nbh * tp.nkc;
+   const int e_phase = geo + nh + 6 * 28 + tp.nbh * tp.nkc;
    const int dbl = std::max(cell_phase, e_phase);
    y.o_int = 8 * dbl;
    y.bytes = y.o_int + 4 * 128;
@@ -343,15 +344,24 @@ static FmLayout fm_layout(const TypePlan &tp, bool fix)
 // at most 128 VGPRs (4 wavefronts per SIMD, with the LDS budget above): the
 // kernel waits on memory half of its time, so wavefronts in flight pay more
 // than the few values it then keeps in scratch
-template <bool FIX>
+//
+// PAIR (at most 32 cells, at most 256 Hermite functions): two primitive
+// quartets per wavefront.  Lanes 0-31 take the cells of the first, lanes
+// 32-63 those of the second; their survivors share the G_m and Hermite
+// batches and each quartet keeps its own cell sum and transform.
+template <bool FIX, bool PAIR>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_fm_r(const TypePlan *__restrict__ tpp, DevTables t, BatchDev b, FmLayout lay)
 {
+   static_assert(!(FIX && PAIR), "the fix-up takes one slot per wavefront");
+   constexpr int NPR = PAIR ? 2 : 1;      // primitive quartets per wavefront
+   constexpr int NACC = PAIR ? 4 : 8;     // Hermite functions per lane and quartet, / 64
    extern __shared__ double lds[];
    const TypePlan &tp = *tpp;
    const int lane = threadIdx.x;
+   const int half = PAIR ? lane >> 5 : 0;   // the lane's quartet
    const int L = tp.L, nh = tp.nh, Lp1 = L + 1;
    const int SB = lay.SB;
-   double *G = lds;
+   double *G = lds + GEO * half;      // geometry of the lane's quartet
    double *fmv = lds + lay.o_fmv;     // [64][L+1]
    double *sv = lds + lay.o_sv;       // [64][5]
    double *Rb = lds + lay.o_R;        // [2][SB][nh]
@@ -360,21 +370,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
    const uint32_t nwork = FIX ? min(b.fix_count[0], b.fix_cap) : (b.p1 - b.p0);
    const uint32_t *__restrict__ e1 = b.e1, *__restrict__ e2 = b.e2;
 
-   for (uint32_t w = blockIdx.x; w < nwork; w += gridDim.x) {
+   for (uint32_t w0 = blockIdx.x * NPR; w0 < nwork; w0 += gridDim.x * NPR) {
+      const uint32_t w = w0 + half;
+      const bool hvalid = w < nwork;                    // the lane's quartet exists
+      const int nprim = (int)min((uint32_t)NPR, nwork - w0);
       uint32_t p;
       int fix_n3 = 0;
       if (FIX) {
          p = b.fix[2 * w];
          fix_n3 = (int)(b.fix[2 * w + 1] & 0xff);
       } else {
-         p = b.p0 + w;
+         p = b.p0 + (hvalid ? w : w0);
       }
       const uint32_t r = b.prm_row[p];
       const RowRec row = b.rows[r];
       const GroupRec &g = b.groups[row.group];
       const uint32_t ip = b.prm_ip[p];
       const int n1 = row.n12 & 0xffff, n2 = row.n12 >> 16;
-      if (lane == 0) {
+      if ((lane & (64 / NPR - 1)) == 0 && hvalid) {
          const double *data = b.data;
          PrimGeo geo;
          prim_prepare(data + g.R[0], data + g.R[1], data + g.R[2], data + g.R[3], data[g.Z[0] + (ip & 15)],
@@ -393,18 +406,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
          G[G_I2CD] = geo.i2cd;
       }
       __syncthreads();
-      double acc[8];
+      double acc[NPR][NACC];
 #pragma unroll
-      for (int i = 0; i < 8; i++) acc[i] = 0.0;
+      for (int q = 0; q < NPR; q++)
+#pragma unroll
+         for (int i = 0; i < NACC; i++) acc[q][i] = 0.0;
       double quirk = 0.0;
       unsigned long long n_surv = 0, csum = 0, n_any = 0;
-      bool any_surv = false;
+      bool any_surv[NPR];
+#pragma unroll
+      for (int q = 0; q < NPR; q++) any_surv[q] = false;
       const unsigned all = (1u << tp.nL) - 1u;
       const int ncell = FIX ? 1 : t.Ng;
 
       for (int base = 0; base < ncell; base += 64) {
-         const int n3 = FIX ? fix_n3 : base + lane;
-         const bool valid = FIX ? (lane == 0) : (n3 < t.Ng);
+         const int n3 = FIX ? fix_n3 : base + (PAIR ? (lane & 31) : lane);
+         const bool valid = FIX ? (lane == 0) : (hvalid && n3 < t.Ng);
          int status = 0;  // 1 kept by every class, 2 screened by every class, 3 class-dependent
          unsigned mask = 0;
          double *myf = fmv + lane * Lp1;
@@ -511,16 +528,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
          // survivors of this pass, in lane order
          const unsigned long long smask = __ballot(status == 1);
          const int nsv = __popcll(smask);
+         const int nsv0 = PAIR ? __popcll(smask & 0xffffffffull) : nsv;   // survivors of the first quartet
          if (status == 1) svl[__popcll(smask & ((1ull << lane) - 1ull))] = lane;
          __syncthreads();
          if (nsv == 0) continue;
-         any_surv = true;
+         any_surv[0] = any_surv[0] || nsv0 > 0;
+         if (PAIR) any_surv[NPR - 1] = any_surv[NPR - 1] || nsv > nsv0;
          // G_1..G_L of the patch survivors, one (survivor, order) per lane
          {
-            const double Kfac = G[G_KFAC];
             for (int idx = lane; idx < nsv * L; idx += 64) {
                const int s = idx / L, m = idx - s * L + 1;
                const int ln = svl[s];
+               const double Kfac = lds[GEO * (PAIR ? ln >> 5 : 0) + G_KFAC];
                const int info = svi[ln];
                if ((info & 3) == GINT_G_PATCH) {
                   double T1[14];
@@ -532,12 +551,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
          }
          __syncthreads();
          {
-            const double m2rho = -2.0 * G[G_RHO];
             for (int idx = lane; idx < nsv * Lp1; idx += 64) {
                const int s = idx / Lp1, m = idx - s * Lp1;
+               const int ln = svl[s];
+               const double m2rho = -2.0 * lds[GEO * (PAIR ? ln >> 5 : 0) + G_RHO];
                double pw = 1.0;
                for (int k = 0; k < m; k++) pw *= m2rho;
-               fmv[svl[s] * Lp1 + m] *= pw;
+               fmv[ln * Lp1 + m] *= pw;
             }
          }
          __syncthreads();
@@ -567,24 +587,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                prv = cur;
                cur = tmp;
             }
-            // the n = 0 layer goes straight into the cell sum
+            // the n = 0 layer goes straight into the cell sum of its quartet
 #pragma unroll
-            for (int i = 0; i < 8; i++) {
+            for (int i = 0; i < NACC; i++) {
                const int h = lane + 64 * i;
                if (h < nh) {
-                  double a = acc[i];
+                  double a0 = acc[0][i], a1 = acc[NPR - 1][i];
                   if (h == 0) {
-                     for (int s = 0; s < nb; s++) a += fmv[svl[s0 + s] * Lp1];
+                     for (int s = 0; s < nb; s++) {
+                        const double v = fmv[svl[s0 + s] * Lp1];
+                        if (!PAIR || s0 + s < nsv0) a0 += v;
+                        else a1 += v;
+                     }
                   } else {
                      const HermEntry e = c_herm.e[h];
                      const double cf = (double)e.coef;
                      for (int s = 0; s < nb; s++) {
                         double v = sv[svl[s0 + s] * 5 + e.dir] * prv[s * nh + e.m1];
                         if (e.m2 >= 0) v += cf * prv[s * nh + e.m2];
-                        a += v;
+                        if (!PAIR || s0 + s < nsv0) a0 += v;
+                        else a1 += v;
                      }
                   }
-                  acc[i] = a;
+                  acc[0][i] = a0;
+                  if (PAIR) acc[NPR - 1][i] = a1;
                }
             }
             __syncthreads();
@@ -592,31 +618,39 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       }
 
       // ---- Hermite -> cartesian, once per primitive quartet ----
-      double *sR = lds + GEO;            // [nh]
+      double *sR = lds + lay.o_fmv;      // [nh]
       double *Ex = sR + nh;              // [3][28] bra E^{e0}_t
       double *Fk = Ex + 84;              // [3][28] ket (-1)^tau E^{f0}_tau
       double *W = Fk + 84;               // [nbh][nkc]
-      for (int off = 32; off > 0; off >>= 1) quirk += __shfl_down(quirk, off, 64);
-      quirk = __shfl(quirk, 0, 64);
-      double *dst = FIX ? b.fix_ws + (size_t)w * (tp.ncp + 16) : b.ac0 + (size_t)(p - b.p0) * tp.ncp;
-      if (!FIX && !any_surv) {
-         // no cell survives: the blocks are zero but for the screened slots'
-         // Fm[0] in (00|00)
-         for (int j = lane; j < tp.ncp; j += 64) dst[j] = (j == 0 && tp.has00) ? quirk : 0.0;
-         __syncthreads();
-      } else {
+      // screened slots' Fm[0], summed per quartet (lane 0, and lane 32 for the second)
+      for (int off = 32 / NPR; off > 0; off >>= 1) quirk += __shfl_down(quirk, off, 64 / NPR);
+      double qsum[NPR];
+#pragma unroll
+      for (int q = 0; q < NPR; q++) qsum[q] = __shfl(quirk, 32 * q, 64);
+#pragma unroll
+      for (int q = 0; q < NPR; q++) {
+         if (q >= nprim) break;
+         const double *Gq = lds + GEO * q;
+         double *dst = FIX ? b.fix_ws + (size_t)w * (tp.ncp + 16) : b.ac0 + (size_t)(w0 + q) * tp.ncp;
+         if (!FIX && !any_surv[q]) {
+            // no cell survives: the blocks are zero but for the screened
+            // slots' Fm[0] in (00|00)
+            for (int j = lane; j < tp.ncp; j += 64) dst[j] = (j == 0 && tp.has00) ? qsum[q] : 0.0;
+            __syncthreads();
+            continue;
+         }
          __syncthreads();
 #pragma unroll
-         for (int i = 0; i < 8; i++) {
+         for (int i = 0; i < NACC; i++) {
             const int h = lane + 64 * i;
-            if (h < nh) sR[h] = acc[i];
+            if (h < nh) sR[h] = acc[q][i];
          }
          if (lane < 6) {
             const int d = lane % 3;
             const bool ket = lane >= 3;
             const int Lm = ket ? tp.LC : tp.LA;
-            const double x = ket ? G[G_QC + d] : G[G_PA + d];
-            const double i2 = ket ? G[G_I2CD] : G[G_I2AB];
+            const double x = ket ? Gq[G_QC + d] : Gq[G_PA + d];
+            const double i2 = ket ? Gq[G_I2CD] : Gq[G_I2AB];
             double *E = (ket ? Fk : Ex) + d * 28;
             E[0] = 1.0;
             for (int e = 0; e < Lm; e++) {
@@ -672,7 +706,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                   }
                }
             const int jd = (int)(it >> 16);
-            if (!FIX && jd == 0 && tp.has00) s += quirk;
+            if (!FIX && jd == 0 && tp.has00) s += qsum[q];
             dst[jd] = s;
          }
          __syncthreads();
@@ -692,7 +726,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                atomicAdd(&st[0], n_surv);
                atomicAdd(&st[1], csum);
             }
-            atomicAdd(&st[2], n_any * (unsigned long long)tp.surv_flops + (unsigned long long)tp.prim_flops);
+            atomicAdd(&st[2], n_any * (unsigned long long)tp.surv_flops + (unsigned long long)(tp.prim_flops * nprim));
          }
       }
    }
@@ -1143,11 +1177,20 @@ size_t fm_r_lds_bytes(const TypePlan &tp)
 int launch_fm_r(const TypePlan &tp, const DevTables &t, const BatchDev &b, void *stream)
 {
    if (b.p1 <= b.p0) return 0;
+   const uint32_t n = b.p1 - b.p0;
+   // two primitive quartets per wavefront when each takes at most half of
+   // the lanes' cells and its cell sum fits 4 values per lane
+   const FmLayout yp = fm_layout(tp, false, true);
+   if (t.Ng <= 32 && tp.nh <= 256 && (size_t)yp.bytes <= LDS_LIMIT) {
+      // workgroups stay resident and stride over the pairs
+      const unsigned grid = std::min((n + 1) / 2, 16384u);
+      hipLaunchKernelGGL((k_fm_r<false, true>), dim3(grid), dim3(64), yp.bytes, (hipStream_t)stream, b.tp, t, b, yp);
+      return hipGetLastError() == hipSuccess ? 0 : -1;
+   }
    const FmLayout y = fm_layout(tp, false);
    if ((size_t)y.bytes > LDS_LIMIT) return -3;
-   // workgroups stay resident and stride over the primitive quartets
-   const unsigned grid = std::min(b.p1 - b.p0, 16384u);
-   hipLaunchKernelGGL(k_fm_r<false>, dim3(grid), dim3(64), y.bytes, (hipStream_t)stream, b.tp, t, b, y);
+   const unsigned grid = std::min(n, 16384u);
+   hipLaunchKernelGGL((k_fm_r<false, false>), dim3(grid), dim3(64), y.bytes, (hipStream_t)stream, b.tp, t, b, y);
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1156,7 +1199,7 @@ int launch_fixup(const TypePlan &tp, const DevTables &t, const BatchDev &b, uint
    const FmLayout y = fm_layout(tp, true);
    if ((size_t)y.bytes > LDS_LIMIT) return -3;
    const unsigned grid = std::max(1u, std::min(n_fix, 4096u));
-   hipLaunchKernelGGL(k_fm_r<true>, dim3(grid), dim3(64), y.bytes, (hipStream_t)stream, b.tp, t, b, y);
+   hipLaunchKernelGGL((k_fm_r<true, false>), dim3(grid), dim3(64), y.bytes, (hipStream_t)stream, b.tp, t, b, y);
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
