@@ -1,7 +1,8 @@
 // gfx950 kernels of the K-build.  A launch covers one chunk of one quartet
 // type (the shell structure of the four sets of a set quartet).
 //
-//   k_fm_r         one wavefront per primitive quartet.  Lanes evaluate
+//   k_fm_r         one wavefront per primitive quartet (two when the cells
+//                  fit half the lanes).  Lanes evaluate
 //                  G_m(R,T) (or F_m) of 64 lattice cells at once and decide
 //                  Fm[0] > 1e-12 for every class L of the type; surviving
 //                  cells run the McMurchie-Davidson Hermite recursion
@@ -12,12 +13,12 @@
 //                  classes at once, reference src/compute_Fm.cpp (prepare,
 //                  Fm, Vm), src/compute_VRR.cpp and the cell sum of
 //                  src/compute_ECO.cpp:216-272.
-//   k_fm_r<FIX>    slots whose screening differs between the classes of the
+//   k_fm_r<true,.> slots whose screening differs between the classes of the
 //                  type (Fm[0] within rounding of 1e-12 at different L): the
-//                  slot's blocks alone, added by k_cqs to the classes that
-//                  keep it.
-//   k_cqs          one workgroup per (set quartet, class), a batch of its
-//                  contractions at once: contraction over primitives
+//                  slot's blocks alone, added by the consumers to the classes
+//                  that keep it.
+//   k_gqs          one workgroup per set quartet, its classes in turn, a
+//                  batch of contractions at once: contraction over primitives
 //                  (src/compute_ECO.cpp:12-214), HRR of each (n1,n2) row
 //                  (src/compute_HRR.cpp) summed on chip, cartesian ->
 //                  spherical (src/compute_SPH.cpp:58-237), and the exchange
