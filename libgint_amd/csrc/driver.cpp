@@ -109,6 +109,7 @@ struct GdrvParams {
    int symmetric;      // 1: unique quartets with weights; 0: every ordered quartet, weight 1/4
    double cell_A;      // cell edge; 0 -> box_A (a larger cell makes the minimum image the identity)
    int np_tm;          // primitives of the spdf centres (16 in the preset workloads; fewer for tests)
+   int max_cells;      // > 0: keep only the first max_cells lattice vectors (nearest first)
 };
 
 struct GdrvSystem {
@@ -189,6 +190,7 @@ void build(GdrvSystem &S)
       std::stable_sort(v.begin(), v.end(), [](const std::array<int, 3> &x, const std::array<int, 3> &y) {
          return x[0] * x[0] + x[1] * x[1] + x[2] * x[2] < y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
       });
+      if (p.max_cells > 0 && (size_t)p.max_cells < v.size()) v.resize(p.max_cells);
       S.neighs.clear();
       for (auto &g : v)
          for (int k = 0; k < 3; k++) S.neighs.push_back(g[k] * cedge);

@@ -79,6 +79,10 @@ CASES = [
     ("water2_pbc27_fixup", dict(n_water=2, box_A=7.0, image_shells=1, np_max=3, seed=13), dict(debug=1)),
     ("water1_pbc125_coulomb", dict(n_water=1, box_A=6.0, image_shells=2, basis=1, np_max=3, seed=14),
      dict(truncated=False)),
+    # 32 cells: the last count at which a wavefront gates two primitive
+    # quartets at once; 33: one quartet per wavefront
+    ("water2_cells32", dict(n_water=2, box_A=6.0, image_shells=2, max_cells=32, basis=1, np_max=3, seed=16), {}),
+    ("water2_cells33", dict(n_water=2, box_A=6.0, image_shells=2, max_cells=33, basis=1, np_max=3, seed=16), {}),
     # an spdf centre (2s 2p 2d 4f contractions): f-shell classes, four-fold
     # contracted shells and quartet types up to L = 12, with 2 primitives so
     # that the oracle finishes in seconds
