@@ -373,10 +373,13 @@ __global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, 
                for (int c = 0; c < NCOL; c++)
 #pragma unroll
                   for (int q = 0; q < TQ_PB; q++) av[c][q] = (cv[c] && q < nb) ? A0[(size_t)(p0 + q) * ncpt + srcv[c]] : 0.0;
+               // the batch's primitive indices, one per lane, loaded with the blocks
+               const int lq8 = lane & (TQ_PB - 1);
+               const uint32_t ipl = lq8 < nb ? IP[p0 + lq8] : 0u;
                __syncthreads();
                // coefficient products of the batch, once per (primitive, contraction)
                for (int q = 0; q < nb; q++) {
-                  const uint32_t ip = IP[p0 + q];
+                  const uint32_t ip = (uint32_t)__shfl((int)ipl, q, 64);
                   for (int lq = lane; lq < nqc; lq += 64) {
                      const uint32_t cd = code[l0 + lq];
                      kpb[q * s + lq] = co[(cd & 255) * 16 + (ip & 15)] * co[64 + ((cd >> 8) & 255) * 16 + ((ip >> 4) & 15)] *
