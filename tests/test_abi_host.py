@@ -141,3 +141,22 @@ def test_fortran_binding_links(tmp_path):
     nm = subprocess.run(["nm", "-u", str(out)], capture_output=True, text=True).stdout
     used = set(re.findall(r"\b(libgint_\w+)", nm))
     assert len(used) >= 20, used
+
+
+def test_synthetic_cell_list_cap():
+    """max_cells keeps the nearest lattice vectors of the full list, in
+    order: 27 = the 3x3x3 block, 33 = that block plus the six |g|^2 = 4 cells"""
+    from libgint_amd import driver as D
+
+    full = D.System(n_water=1, box_A=6.0, image_shells=2, basis=1, np_max=2, seed=1)
+    gf = full.neighs().reshape(-1, 3)
+    assert full.Ng == 125
+    edge = np.max(np.abs(gf))
+    for mc in (27, 32, 33):
+        s = D.System(n_water=1, box_A=6.0, image_shells=2, basis=1, np_max=2, seed=1, max_cells=mc)
+        g = s.neighs().reshape(-1, 3)
+        assert s.Ng == mc
+        np.testing.assert_array_equal(g, gf[:mc])
+    r2 = np.rint(np.sum((gf / (edge / 2)) ** 2, axis=1)).astype(int)
+    assert np.all(np.diff(r2) >= 0)
+    assert np.all(r2[:27] <= 3) and np.all(r2[27:33] == 4)
