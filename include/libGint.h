@@ -100,7 +100,7 @@ void libgint_amd_enable_stats(void *handle, int on);
 void libgint_amd_stats(void *handle, uint64_t out[8]);
 /* per-kernel device time with HIP events on the handle's stream */
 void libgint_amd_enable_timing(void *handle, int on);
-/* out[0..3] ms in fm_r / the thread-per-contraction quartet kernels tq /
+/* out[0..3] ms in fm_r / the wavefront-per-set-quartet kernels tq /
    the per-set-quartet quartet kernel gqs / fix-up, out[4..7] launches,
    out[9], out[10] algorithmic FLOPs of tq and gqs (out[8] and out[11] = 0:
    the fm_r FLOPs are libgint_amd_stats out[6]) */

@@ -206,7 +206,8 @@ struct CqLayout {
 // launchers (kernels.hip), asynchronous on `stream`; 0 or a negative code
 int launch_fm_r(const TypePlan &tp, const DevTables &t, const BatchDev &b, void *stream);
 int launch_fixup(const TypePlan &tp, const DevTables &t, const BatchDev &b, uint32_t n_fix, void *stream);
-// thread-per-contraction kernels of the small classes (kernels_tq.hip)
+// one-wavefront-per-set-quartet kernels of the classes with at most 128
+// block columns (kernels_tq.hip)
 bool tq_has(int code);
 int launch_tq(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b, const KTargets &kt,
               const int *srcmap, void *stream);

@@ -159,8 +159,8 @@ class Engine {
    void ensure_side();
    int my_thr_ = 0, nomp_ = 1;
    void *stream_ = nullptr;
-   // the consumers of a chunk's blocks only add into K: the thread-per-
-   // contraction kernels run on side streams beside gqs on stream_, and
+   // the consumers of a chunk's blocks only add into K: the small-class
+   // kernels run on side streams beside gqs on stream_, and
    // stream_ waits for them before the blocks are overwritten
    static constexpr int NSIDE = 3;
    void *side_[NSIDE] = {};

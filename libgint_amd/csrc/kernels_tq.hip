@@ -574,7 +574,7 @@ int launch_one(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev
 
 }  // namespace tq
 
-// the classes with a thread-per-contraction kernel
+// the classes with a wavefront-per-set-quartet kernel
 #define GINT_TQ_CLASSES(X)                                                                                      \
    X(0, 0, 0, 0) X(1, 0, 0, 0) X(0, 0, 1, 0) X(0, 1, 0, 0) X(0, 0, 0, 1) X(2, 0, 0, 0) X(0, 0, 2, 0)          \
    X(1, 1, 0, 0) X(1, 0, 1, 0) X(0, 0, 1, 1) X(0, 2, 0, 0) X(0, 0, 0, 2) X(1, 0, 0, 1) X(0, 1, 1, 0)          \
