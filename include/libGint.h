@@ -127,8 +127,25 @@ void libgint_amd_plan_info(void *handle, uint64_t out[6]);
 /* work recorded and not yet on the device (host only, no GPU call):
    out[0] quartet types, [1] set quartets, [2] rows, [3] primitive quartets,
    [4] (set quartet, class) pairs with add_qrtt calls, [5] of them kept as
-   explicit lists because their offsets are not affine */
-void libgint_amd_queue_info(void *handle, uint64_t out[6]);
+   explicit lists because their offsets are not affine, [6] set quartets
+   whose calls reduce to one set-level record (libgint_amd_add_set_blocks),
+   [7] bytes of the largest set quartet's primitive blocks */
+void libgint_amd_queue_info(void *handle, uint64_t out[8]);
+/* Every add_qrt / add_qrtt call of the current set quartet (i,j,k,l) at
+   once, for a caller whose P / K blocks are laid out per pair of sets, as
+   CP2K's block matrices are: element (f, g) of the (a,c) pair block, f a
+   spherical function of set i and g one of set k, is at
+   off_ac + (Tac ? f * ld_ac + g : g * ld_ac + f), likewise for (a,d), (b,c)
+   and (b,d); the functions of a set run shell by shell in libgint_set_Atom_L
+   order, contraction by contraction, 2l+1 each.  Equivalent to add_qrt and
+   all nla*nlb*nlc*nld add_qrtt calls of every class of the set quartet with
+   those offsets and symm_fac; call it after the set quartet's add_shell
+   calls, then libgint_add_set.  A per-call stream with this layout is
+   recognised at libgint_add_set and stored the same way (32 bytes per set
+   quartet). */
+void libgint_amd_add_set_blocks(void *handle, double symm_fac, int ld_ac, int ld_ad, int ld_bc, int ld_bd,
+                                unsigned int off_ac, unsigned int off_ad, unsigned int off_bc, unsigned int off_bd,
+                                bool Tac, bool Tad, bool Tbc, bool Tbd);
 /* reads CP2K's truncated-Coulomb coefficient file (the T_C_G_DATA
    t_c_g.dat) as reference src/t_c_g0_n.cpp:29-51 (read_c0) does: 207
    patches of 22 x 105 white-space separated numbers, patch after patch.

@@ -9,6 +9,7 @@ import hashlib
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -24,6 +25,8 @@ REF_SRC = "/root/reference/src"
 
 HOST_SRC = ["engine.cpp", "abi.cpp", "plan.cpp", "boys_table.cpp", "c0_file.cpp"]
 DEV_SRC = ["kernels.hip", "kernels_tq.hip"]
+# kernels_tq_part.hip is built once per part of the templated class kernels
+TQ_PARTS = 8
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 HOSTFLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
@@ -58,17 +61,24 @@ def tcg_sha256():
         return hashlib.sha256(f.read()).hexdigest()
 
 
-def build_lib(verbose=False):
+def build_lib(verbose=False, jobs=None):
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    steps = []
     for src in HOST_SRC:
         o = os.path.join(BUILD, src + ".o")
-        _run(["g++"] + HOSTFLAGS + ["-c", os.path.join(CSRC, src), "-o", o], verbose)
-        objs.append(o)
+        steps.append((["g++"] + HOSTFLAGS + ["-c", os.path.join(CSRC, src), "-o", o], o))
     for src in DEV_SRC:
         o = os.path.join(BUILD, src + ".o")
-        _run([HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", o], verbose)
-        objs.append(o)
+        steps.append(([HIPCC] + HIPFLAGS + ["-c", os.path.join(CSRC, src), "-o", o], o))
+    for part in range(TQ_PARTS):
+        o = os.path.join(BUILD, "kernels_tq_part%d.o" % part)
+        steps.append(([HIPCC] + HIPFLAGS + ["-DGINT_TQ_PART=%d" % part, "-c", os.path.join(CSRC, "kernels_tq_part.hip"),
+                       "-o", o], o))
+    # the compiles are independent: run them side by side (at most 16 jobs)
+    jobs = jobs or min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda st: _run(st[0], verbose), steps))
+    objs = [o for _, o in steps]
     tmp = LIB + ".tmp"
     _run(["g++", "-shared", "-fopenmp", "-o", tmp] + objs +
          ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")], verbose)

@@ -96,6 +96,14 @@ void libgint_add_qrtt(void *h, double symm_fac, int la, int lb, int lc, int ld, 
                   off_bc, off_bd, Tac, Tad, Tbc, Tbd);
 }
 void libgint_add_set(void *h) { E(h)->add_set(); }
+void libgint_amd_add_set_blocks(void *h, double symm_fac, int ld_ac, int ld_ad, int ld_bc, int ld_bd, unsigned off_ac,
+                                unsigned off_ad, unsigned off_bc, unsigned off_bd, bool Tac, bool Tad, bool Tbc, bool Tbd)
+{
+   const int ld[4] = {ld_ac, ld_ad, ld_bc, ld_bd};
+   const unsigned off[4] = {off_ac, off_ad, off_bc, off_bd};
+   const bool T[4] = {Tac, Tad, Tbc, Tbd};
+   E(h)->add_set_blocks(symm_fac, ld, off, T);
+}
 void libgint_dispatch(void *h) { E(h)->dispatch_all(); }
 
 void libgint_amd_destroy_handle(void *h) { delete static_cast<Engine *>(h); }
@@ -120,7 +128,7 @@ void libgint_amd_plan_end(void *h) { E(h)->plan_end(); }
 void libgint_amd_plan_execute(void *h) { E(h)->plan_execute(); }
 void libgint_amd_plan_clear(void *h) { E(h)->plan_clear(); }
 void libgint_amd_plan_info(void *h, uint64_t out[6]) { E(h)->plan_info(out); }
-void libgint_amd_queue_info(void *h, uint64_t out[6]) { E(h)->queue_info(out); }
+void libgint_amd_queue_info(void *h, uint64_t out[8]) { E(h)->queue_info(out); }
 int libgint_amd_math_selftest(const double *x, const double *y, int n, double *out)
 {
    return gint::device_math_selftest(x, y, n, out);

@@ -14,6 +14,12 @@
 
 #include <vector>
 
+#if defined(__HIPCC__)
+#define GINT_HDI __host__ __device__ inline
+#else
+#define GINT_HDI inline
+#endif
+
 namespace gint {
 
 constexpr int MAXL = 3;          // highest angular momentum of a shell (f)
@@ -49,7 +55,9 @@ struct GroupRec {
    uint32_t Kof[4][MAXSH];    // normalised coefficients of shell s of centre t (nl x np)
    uint32_t np4;              // np-1 of the four sets, 4-bit fields
    uint32_t set_ij, set_kl;   // set indices, 16-bit fields (survivor checksum only)
-   uint32_t pad[3];
+   uint32_t ks;               // KS_SETLEVEL: its KsSet; else its first KsGroup (nclass of them)
+   uint32_t ksmode;           // KS_SETLEVEL or 0
+   uint32_t pad;
 };
 static_assert(sizeof(GroupRec) == 128, "GroupRec layout");
 
@@ -68,7 +76,26 @@ struct KsGroup {
 static_assert(sizeof(KsGroup) == 64, "KsGroup layout");
 
 constexpr uint32_t KS_EXPLICIT = 1u << 8;
-inline uint32_t ks_count(uint32_t flags) { return flags >> 16; }
+GINT_HDI uint32_t ks_count(uint32_t flags) { return flags >> 16; }
+
+// Every add_qrtt call of one set quartet at once, when the caller lays out
+// its P / K blocks per pair of sets (the layout of CP2K's block matrices):
+// function f of set x and function g of set y sit at
+// base + (T ? f ld + g : g ld + f) of the (x,y) pair block, with the
+// functions of a set ordered shell by shell, contraction by contraction,
+// 2l+1 each.  Every class of the set quartet has all its contraction
+// combinations, and they share leading dimensions, transposes and the
+// symmetry code.  The KsGroup of each class follows from it (ks_from_set).
+struct KsSet {
+   uint32_t base[4];   // ac ad bc bd pair blocks: element of (function 0, function 0)
+   uint32_t ld01;      // ld_ac | ld_ad << 16
+   uint32_t ld23;      // ld_bc | ld_bd << 16
+   uint32_t flags;     // bits 0-3 transposes, 4-6 symmetry code
+   uint32_t pad;
+};
+static_assert(sizeof(KsSet) == 32, "KsSet layout");
+
+constexpr uint32_t KS_SETLEVEL = 1u;   // GroupRec::ksmode
 
 // one add_qrtt call, kept when a (set quartet, class) is not affine
 struct KsRec {
@@ -125,6 +152,8 @@ struct TypePlan {
    int cls_eoff[MAXTC];      // offset of class k inside a row's contracted doubles
    int co_off[4][MAXSH];     // quartet kernel: coefficients of shell s of centre t in LDS, nl x 16
    int nco_tot;
+   int fun0[4][MAXSH];       // first spherical function of shell s within set t (sets list shells in order)
+   int nfun[4];              // spherical functions of set t
    unsigned long long cls_mult[MAXTC];   // checksum multiplier of class k (odd)
    unsigned long long mult_all;          // sum of cls_mult
    // algorithmic FLOPs of this library's kernels (Hermite path)
@@ -132,6 +161,38 @@ struct TypePlan {
    long long prim_flops;     // per primitive quartet: Hermite -> cartesian transform
    long long ref_slot_flops; // per slot, summed over classes: the reference's Fm + VRR
 };
+
+// The KsGroup of class k of a set quartet stored as a KsSet: offsets of
+// contraction (ia,ib,ic,id) are base + ia step[0] + ic step[1] (ac), ...
+GINT_HDI KsGroup ks_from_set(const KsSet &s, const TypePlan &tp, int k)
+{
+   KsGroup g;
+   const int sh = tp.cls_sh[k], code = tp.cls_code[k];
+   const int sa = sh & 15, sb = (sh >> 4) & 15, sc = (sh >> 8) & 15, sd = (sh >> 12) & 15;
+   const long long nsa = 2 * (code / 64) + 1, nsb = 2 * ((code / 16) % 4) + 1, nsc = 2 * ((code / 4) % 4) + 1,
+                   nsd = 2 * (code % 4) + 1;
+   const long long fa = tp.fun0[0][sa], fb = tp.fun0[1][sb], fc = tp.fun0[2][sc], fd = tp.fun0[3][sd];
+   const long long ld[4] = {s.ld01 & 0xffff, s.ld01 >> 16, s.ld23 & 0xffff, s.ld23 >> 16};
+   // element (row r, column c) of pair block q: rows of a (ac, ad) or b (bc, bd)
+   auto pos = [&](int q, long long r, long long c) { return ((s.flags >> q) & 1) ? r * ld[q] + c : c * ld[q] + r; };
+   g.base[0] = (uint32_t)(s.base[0] + pos(0, fa, fc));
+   g.base[1] = (uint32_t)(s.base[1] + pos(1, fa, fd));
+   g.base[2] = (uint32_t)(s.base[2] + pos(2, fb, fc));
+   g.base[3] = (uint32_t)(s.base[3] + pos(3, fb, fd));
+   g.step[0] = (int32_t)pos(0, nsa, 0);
+   g.step[1] = (int32_t)pos(0, 0, nsc);
+   g.step[2] = (int32_t)pos(1, nsa, 0);
+   g.step[3] = (int32_t)pos(1, 0, nsd);
+   g.step[4] = (int32_t)pos(2, nsb, 0);
+   g.step[5] = (int32_t)pos(2, 0, nsc);
+   g.step[6] = (int32_t)pos(3, nsb, 0);
+   g.step[7] = (int32_t)pos(3, 0, nsd);
+   g.ld01 = s.ld01;
+   g.ld23 = s.ld23;
+   g.flags = (s.flags & 0x7fu) | (uint32_t)tp.cls_nl[k] << 16;
+   g.expl = 0;
+   return g;
+}
 
 // Device-resident constant tables.
 struct DevTables {
@@ -158,7 +219,8 @@ struct BatchDev {
    const GroupRec *groups;
    const uint32_t *prm_row;   // row of each primitive quartet
    const uint32_t *prm_ip;    // ipa | ipb << 4 | ipc << 8 | ipd << 12
-   const KsGroup *ksg;        // [group][class]
+   const KsGroup *ksg;        // per class of the groups that are not KS_SETLEVEL (GroupRec::ks)
+   const KsSet *kss;          // KS_SETLEVEL groups
    const KsRec *ksx;
    const double *data;
    double *ac0;               // [primitive - p0][ncp]
@@ -167,11 +229,18 @@ struct BatchDev {
    uint32_t g0, g1;           // groups
    unsigned long long *stats; // STATS_SLOTS x {surviving (slot, class) pairs, checksum, K1 FLOPs} (nullable)
    uint32_t *fix;             // mixed-gate slots: (primitive, n3 | mask << 8)
-   uint32_t *fix_count;       // [0] entries, [1] overflow
+   uint32_t *fix_count;       // [0] entries, [1] slots beyond fix_cap (kept by every class instead)
    uint32_t fix_cap;
    int force_fix;             // route every surviving slot through the fix-up path (test hook)
    double *fix_ws;            // fix-up workspace, ncp doubles per entry
 };
+
+// the K layout of class k of group g
+GINT_HDI KsGroup group_ks(const BatchDev &b, const TypePlan &tp, const GroupRec &g, int k)
+{
+   if (g.ksmode & KS_SETLEVEL) return ks_from_set(b.kss[g.ks], tp, k);
+   return b.ksg[(size_t)g.ks + k];
+}
 
 struct KTargets {
    const double *P[2];
@@ -204,6 +273,9 @@ struct CqLayout {
 };
 
 // launchers (kernels.hip), asynchronous on `stream`; 0 or a negative code
+// zeroes n words (a kernel, not a memset node, so that the launch sequence
+// can be captured in a graph)
+int launch_zero_u32(uint32_t *p, int n, void *stream);
 int launch_fm_r(const TypePlan &tp, const DevTables &t, const BatchDev &b, void *stream);
 int launch_fixup(const TypePlan &tp, const DevTables &t, const BatchDev &b, uint32_t n_fix, void *stream);
 // one-wavefront-per-set-quartet kernels of the classes with at most 128

@@ -99,7 +99,7 @@ struct GdrvParams {
    int n_water;
    int n_tm;
    double box_A;
-   int image_shells;   // 0: 1 cell, 1: 27, 2: 125, 3: 251 (closed shells |g|^2 <= 13)
+   int image_shells;   // 0: 1 cell, 1: 27 ({-1,0,1}^3), 2: 125 ({-2..2}^3), 3: the 256 shortest lattice vectors
    int basis;          // 0: O 3s2p1d / H 2s1p; 1: O 2s1p / H 1s; 4: O 4s3p2d1f / H 3s2p1d
    int np_min, np_max;
    uint64_t seed;
@@ -182,14 +182,18 @@ void build(GdrvSystem &S)
    }
    {
       std::vector<std::array<int, 3>> v;
+      // the {-r..r}^3 block, nearest first (stable: ties keep the loop
+      // order); image_shells 3 keeps the 256 shortest of {-3..3}^3, the
+      // reference's cell limit (src/define.h:42), which covers |g|^2 <= 14
+      // (251 vectors) and 5 of the 24 at |g|^2 = 17
       const int r = p.image_shells <= 0 ? 0 : std::min(p.image_shells, 3);
       for (int a = -r; a <= r; a++)
          for (int b = -r; b <= r; b++)
-            for (int c = -r; c <= r; c++)
-               if (p.image_shells < 3 || a * a + b * b + c * c <= 13) v.push_back({a, b, c});
+            for (int c = -r; c <= r; c++) v.push_back({a, b, c});
       std::stable_sort(v.begin(), v.end(), [](const std::array<int, 3> &x, const std::array<int, 3> &y) {
          return x[0] * x[0] + x[1] * x[1] + x[2] * x[2] < y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
       });
+      if (p.image_shells >= 3 && v.size() > 256) v.resize(256);
       if (p.max_cells > 0 && (size_t)p.max_cells < v.size()) v.resize(p.max_cells);
       S.neighs.clear();
       for (auto &g : v)
@@ -383,6 +387,10 @@ typedef struct {
    void (*add_set)(void *);
    void (*set_Atom)(void *, int, double *, double *, int);
    void (*set_Atom_L)(void *, int, int, int, double *);
+   // optional: all add_qrt / add_qrtt calls of a set quartet at once
+   // (libgint_amd_add_set_blocks); null when the library has none
+   void (*add_set_blocks)(void *, double, int, int, int, int, unsigned, unsigned, unsigned, unsigned, bool, bool,
+                          bool, bool);
 } GdrvAbi;
 
 namespace {
@@ -517,6 +525,23 @@ void emit_set(const GdrvSystem &S, const GdrvAbi *abi, void *h, int i, int j, in
                                          obc, obd, Tac, Tad, Tbc, Tbd);
                         }
             }
+   abi->add_set(h);
+}
+
+// the same set quartet in one call: the offsets of function 0 of every pair
+// block; the library derives those of each (shell, contraction)
+void emit_set_blocks(const GdrvSystem &S, const GdrvAbi *abi, void *h, int i, int j, int k, int l, int code)
+{
+   static const double fac[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
+   unsigned off[4];
+   int ld[4];
+   bool T[4];
+   block_of(S, i, 0, k, 0, &off[0], &ld[0], &T[0]);
+   block_of(S, i, 0, l, 0, &off[1], &ld[1], &T[1]);
+   block_of(S, j, 0, k, 0, &off[2], &ld[2], &T[2]);
+   block_of(S, j, 0, l, 0, &off[3], &ld[3], &T[3]);
+   abi->add_set_blocks(h, fac[code], ld[0], ld[1], ld[2], ld[3], off[0], off[1], off[2], off[3], T[0], T[1], T[2],
+                       T[3]);
    abi->add_set(h);
 }
 
@@ -666,11 +691,14 @@ void gdrv_stream_info(const GdrvStream *st, int64_t out[5])
 
 void gdrv_stream_free(GdrvStream *st) { delete st; }
 
-// replay sets [set_begin, set_end) with stride into the ABI
+// replay sets [set_begin, set_end) with stride into the ABI; blocks != 0:
+// one add_set_blocks call per set quartet in place of its add_qrt / add_qrtt
+// calls
 void gdrv_stream_replay(const GdrvSystem *S, const GdrvStream *st, const GdrvAbi *abi, void *h, int64_t set_begin,
-                        int64_t set_end, int64_t stride)
+                        int64_t set_end, int64_t stride, int blocks)
 {
    if (stride < 1) stride = 1;
+   if (blocks && !abi->add_set_blocks) blocks = 0;
    set_end = std::min<int64_t>(set_end, (int64_t)st->set_word.size());
    for (int64_t s = set_begin; s < set_end; s += stride) {
       size_t i = (size_t)st->set_word[s];
@@ -686,7 +714,10 @@ void gdrv_stream_replay(const GdrvSystem *S, const GdrvStream *st, const GdrvAbi
             i += 3;
          } else {
             const uint32_t ij = st->w[i + 1], kl = st->w[i + 2];
-            emit_set(*S, abi, h, ij & 0xffff, ij >> 16, kl & 0xffff, kl >> 16, (int)(w & 7));
+            if (blocks)
+               emit_set_blocks(*S, abi, h, ij & 0xffff, ij >> 16, kl & 0xffff, kl >> 16, (int)(w & 7));
+            else
+               emit_set(*S, abi, h, ij & 0xffff, ij >> 16, kl & 0xffff, kl >> 16, (int)(w & 7));
             break;
          }
       }

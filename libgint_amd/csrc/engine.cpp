@@ -74,21 +74,24 @@ void TypeQueue::release()
    free_vec(prm_row);
    free_vec(groups);
    free_vec(ksg);
+   free_vec(kss);
    free_vec(ksx);
 }
 
 void TypeDev::release()
 {
-   void *p[6] = {rows, groups, prm_ip, prm_row, ksg, ksx};
+   void *p[7] = {rows, groups, prm_ip, prm_row, ksg, ksx, kss};
    for (void *x : p)
       if (x) (void)hipFree(x);
    rows = nullptr;
    groups = nullptr;
    prm_ip = prm_row = nullptr;
    ksg = nullptr;
+   kss = nullptr;
    ksx = nullptr;
    for (auto &c : cap) c = 0;
-   n_rows = n_groups = n_prm = n_ksx = 0;
+   n_rows = n_groups = n_prm = n_ksx = n_ksg = n_kss = 0;
+   max_group_bytes = 0;
    chunks.clear();
 }
 
@@ -174,13 +177,17 @@ void Engine::ensure_side()
       if (!ev_join_[i]) HIPCHECK(hipEventCreateWithFlags((hipEvent_t *)&ev_join_[i], hipEventDisableTiming));
 }
 
+// Any table of whole orders is accepted (reference src/libGint.cpp:62-100
+// copies whatever it is given); a quartet type whose total angular momentum
+// exceeds the table's highest order is refused when it is used.
 void Engine::set_potential_truncated(double R_cut, const double *C0, int ld_C0, int C0_size)
 {
-   if (ld_C0 % TCG_NCOEF != 0 || C0_size < ld_C0 * TCG_NPATCH) fatal("set_Potential_Truncated: bad C0 table shape", ld_C0);
-   if (ld_C0 / TCG_NCOEF < MAXLL + 1) fatal("set_Potential_Truncated: the table needs derivative orders 0..12", ld_C0);
+   if (ld_C0 < TCG_NCOEF || ld_C0 % TCG_NCOEF != 0 || (long long)C0_size < (long long)ld_C0 * TCG_NPATCH)
+      fatal("set_Potential_Truncated: C0 must hold 207 patches of ld_C0 = (orders) x 105 coefficients", ld_C0);
    truncated_ = true;
    R_cut_ = R_cut;
    ld_C0_ = ld_C0;
+   nder_ = ld_C0 / TCG_NCOEF - 1;
    C0_.assign(C0, C0 + (size_t)ld_C0 * TCG_NPATCH);
    tables_dirty_ = true;
 }
@@ -270,6 +277,9 @@ int Engine::type_of(int i, int j, int k, int l)
          }
       }
       make_type_plan(ti.plan);
+      if (truncated_ && ti.plan.L > nder_)
+         fatal("the truncated-Coulomb table has fewer derivative orders than this quartet's total angular momentum",
+               ti.plan.L);
       make_type_items(ti.plan, ti.e1, ti.e2);
       if (!fm_r_lds_bytes(ti.plan)) fatal("quartet type exceeds the on-chip budget", ti.plan.L);
       for (int c = 0; c < ti.plan.nclass; c++) {
@@ -367,9 +377,19 @@ void Engine::synchronize()
    HIPCHECK(hipStreamSynchronize((hipStream_t)stream_));
    harvest_events();
    if (d_fixcnt_) {
+      // slots whose screening differs between the classes of a type but
+      // found the fix-up list full were kept by every class: an error of at
+      // most their G_0 ~ 1e-12 times the contraction coefficients in K
       uint32_t c[2];
       HIPCHECK(hipMemcpy(c, d_fixcnt_, sizeof c, hipMemcpyDeviceToHost));
-      if (c[1]) fatal("more slots with class-dependent screening in one chunk than the fix-up list holds", fix_cap_);
+      if (c[1]) {
+         if (!fix_overflow_)
+            std::fprintf(stderr, "libgint: %u slots with class-dependent screening beyond the fix-up list of %zu "
+                         "were kept by every class\n", c[1], fix_cap_);
+         fix_overflow_ += c[1];
+         uint32_t z = 0;
+         HIPCHECK(hipMemcpy(d_fixcnt_ + 1, &z, sizeof z, hipMemcpyHostToDevice));
+      }
    }
 }
 
@@ -411,6 +431,8 @@ void Engine::add_shell(int i, int j, int k, int l, int n1, int n2)
               (uint32_t)(S[3]->np - 1) << 12;
       g.set_ij = (uint32_t)i | (uint32_t)j << 16;
       g.set_kl = (uint32_t)k | (uint32_t)l << 16;
+      g.ks = 0;
+      g.ksmode = 0;
       Q.groups.push_back(g);
       pend_bytes_ += sizeof(GroupRec) + sizeof(KsGroup) * (size_t)types_[type].plan.nclass;
    } else if (cur_sets_ != std::array<int, 4>{i, j, k, l}) {
@@ -540,14 +562,97 @@ KsGroup Engine::compress(std::vector<KsRec> &calls, const TypePlan &tp, int k, s
    return g;
 }
 
+// Whether the per-class groups g[0..nclass) of a set quartet are the
+// set-level layout of one KsSet (every class complete, one leading dimension,
+// transpose and symmetry code per pair block, offsets following the shell
+// order of the sets), and that record.
+bool Engine::set_level(const KsGroup *g, const TypePlan &tp, KsSet &s) const
+{
+   const uint32_t f0 = g[0].flags & 0x7fu;
+   for (int k = 0; k < tp.nclass; k++) {
+      if ((g[k].flags & KS_EXPLICIT) || ks_count(g[k].flags) != (uint32_t)tp.cls_nl[k]) return false;
+      if ((g[k].flags & 0x7fu) != f0 || g[k].ld01 != g[0].ld01 || g[k].ld23 != g[0].ld23) return false;
+   }
+   // class 0 takes shell 0 of every set: its first call is function (0,0) of the pair blocks
+   for (int q = 0; q < 4; q++) s.base[q] = g[0].base[q];
+   s.ld01 = g[0].ld01;
+   s.ld23 = g[0].ld23;
+   s.flags = f0;
+   s.pad = 0;
+   for (int k = 0; k < tp.nclass; k++) {
+      const KsGroup d = ks_from_set(s, tp, k);
+      for (int q = 0; q < 4; q++)
+         if (d.base[q] != g[k].base[q]) return false;
+      const int sh = tp.cls_sh[k];
+      const bool mul[4] = {tp.nl[0][sh & 15] > 1, tp.nl[1][(sh >> 4) & 15] > 1, tp.nl[2][(sh >> 8) & 15] > 1,
+                           tp.nl[3][(sh >> 12) & 15] > 1};
+      // step q moves contraction index of centre ctr[q] (a c a d b c b d)
+      const int ctr[8] = {0, 2, 0, 3, 1, 2, 1, 3};
+      for (int q = 0; q < 8; q++)
+         if (mul[ctr[q]] && d.step[q] != g[k].step[q]) return false;
+   }
+   return true;
+}
+
+void Engine::add_set_blocks(double symm_fac, const int ld[4], const unsigned off[4], const bool T[4])
+{
+   if (!touched_.empty()) fatal("add_set_blocks after add_qrtt calls of the same set quartet", 0);
+   if (cur_type_ < 0) return;   // no row with primitives: nothing to add, as for add_qrtt
+   if (have_blocks_) fatal("add_set_blocks twice for one set quartet", 0);
+   for (int q = 0; q < 4; q++)
+      if (ld[q] < 0 || ld[q] > 0xffff) fatal("add_set_blocks: leading dimension out of 0..65535", ld[q]);
+   KsSet &s = blocks_;
+   for (int q = 0; q < 4; q++) s.base[q] = off[q];
+   s.ld01 = (uint32_t)ld[0] | (uint32_t)ld[1] << 16;
+   s.ld23 = (uint32_t)ld[2] | (uint32_t)ld[3] << 16;
+   s.flags = (uint32_t)T[0] | (uint32_t)T[1] << 1 | (uint32_t)T[2] << 2 | (uint32_t)T[3] << 3 | symm_code(symm_fac) << 4;
+   s.pad = 0;
+   const TypePlan &tp = types_[cur_type_].plan;
+   // the last element of each pair block (rows of a / b, columns of c / d)
+   const int rc[4][2] = {{0, 2}, {0, 3}, {1, 2}, {1, 3}};
+   for (int q = 0; q < 4; q++) {
+      const uint64_t r = (uint64_t)tp.nfun[rc[q][0]] - 1, c = (uint64_t)tp.nfun[rc[q][1]] - 1;
+      const uint64_t top = (uint64_t)off[q] + (T[q] ? r * (uint64_t)ld[q] + c : c * (uint64_t)ld[q] + r);
+      if (top > max_index_) max_index_ = top;
+   }
+   for (int k = 0; k < tp.nclass; k++) n_q_ += (uint64_t)tp.cls_nl[k];
+   have_blocks_ = true;
+}
+
 void Engine::add_set()
 {
    if (cur_type_ >= 0) {
       TypeQueue &Q = q_[cur_type_];
       const TypePlan &tp = types_[cur_type_].plan;
-      Q.groups.back().row1 = (uint32_t)Q.rows.size();
-      for (int k = 0; k < tp.nclass; k++) Q.ksg.push_back(compress(pend_[tp.cls_code[k]], tp, k, Q.ksx));
+      GroupRec &gr = Q.groups.back();
+      gr.row1 = (uint32_t)Q.rows.size();
+      const size_t per_class = sizeof(KsGroup) * (size_t)tp.nclass;
+      if (have_blocks_) {
+         gr.ks = (uint32_t)Q.kss.size();
+         gr.ksmode = KS_SETLEVEL;
+         Q.kss.push_back(blocks_);
+         pend_bytes_ = pend_bytes_ - per_class + sizeof(KsSet);
+      } else {
+         const size_t first = Q.ksg.size();
+         const size_t nx = Q.ksx.size();
+         for (int k = 0; k < tp.nclass; k++) Q.ksg.push_back(compress(pend_[tp.cls_code[k]], tp, k, Q.ksx));
+         KsSet s;
+         if (Q.ksx.size() == nx && set_level(&Q.ksg[first], tp, s)) {
+            Q.ksg.resize(first);
+            gr.ks = (uint32_t)Q.kss.size();
+            gr.ksmode = KS_SETLEVEL;
+            Q.kss.push_back(s);
+            pend_bytes_ = pend_bytes_ - per_class + sizeof(KsSet);
+         } else {
+            if (first > 0xffffffffull - (size_t)tp.nclass) fatal("more than 2^32 class records of one quartet type", 0);
+            gr.ks = (uint32_t)first;
+            gr.ksmode = 0;
+         }
+      }
+   } else if (have_blocks_) {
+      fatal("add_set_blocks without a set quartet", 0);
    }
+   have_blocks_ = false;
    // calls of classes without integrals in this set quartet add nothing
    for (int c : touched_) pend_[c].clear();
    touched_.clear();
@@ -570,6 +675,7 @@ void Engine::dispatch_all()
       Q.prm_row.clear();
       Q.groups.clear();
       Q.ksg.clear();
+      Q.kss.clear();
       Q.ksx.clear();
    }
    pend_bytes_ = 0;
@@ -616,9 +722,9 @@ void Engine::plan_clear()
    plan_dev_.clear();
 }
 
-void Engine::queue_info(uint64_t out[6])
+void Engine::queue_info(uint64_t out[8])
 {
-   for (int i = 0; i < 6; i++) out[i] = 0;
+   for (int i = 0; i < 8; i++) out[i] = 0;
    for (size_t ty = 0; ty < q_.size(); ty++) {
       const TypeQueue &Q = q_[ty];
       if (Q.empty()) continue;
@@ -630,6 +736,13 @@ void Engine::queue_info(uint64_t out[6])
          if (ks_count(g.flags)) out[4]++;
          if (g.flags & KS_EXPLICIT) out[5]++;
       }
+      out[4] += Q.kss.size() * (uint64_t)types_[ty].plan.nclass;
+      out[6] += Q.kss.size();
+      for (const GroupRec &g : Q.groups) {
+         size_t gp = 0;
+         for (uint32_t r = g.row0; r < g.row1; r++) gp += Q.rows[r].n_prm;
+         out[7] = std::max<uint64_t>(out[7], (uint64_t)gp * sizeof(double) * (uint64_t)types_[ty].plan.ncp);
+      }
    }
 }
 
@@ -639,8 +752,8 @@ void Engine::plan_info(uint64_t out[6])
    for (auto &d : plan_dev_) {
       if (d.n_groups == 0) continue;
       nt++;
-      bytes += d.n_rows * sizeof(RowRec) + d.n_groups * sizeof(GroupRec) + d.n_prm * 8 +
-               d.n_groups * (uint64_t)types_[d.type].plan.nclass * sizeof(KsGroup) + d.n_ksx * sizeof(KsRec);
+      bytes += d.n_rows * sizeof(RowRec) + d.n_groups * sizeof(GroupRec) + d.n_prm * 8 + d.n_ksg * sizeof(KsGroup) +
+               d.n_kss * sizeof(KsSet) + d.n_ksx * sizeof(KsRec);
       chunks += d.chunks.size();
       groups += d.n_groups;
       rows += d.n_rows;
@@ -750,15 +863,31 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
       dev_put(D.prm_row, D.cap[3], Q.prm_row);
       dev_put(D.ksg, D.cap[4], Q.ksg);
       dev_put(D.ksx, D.cap[5], Q.ksx);
+      dev_put(D.kss, D.cap[6], Q.kss);
       D.n_groups = Q.groups.size();
       D.n_rows = Q.rows.size();
       D.n_prm = Q.prm_ip.size();
       D.n_ksx = Q.ksx.size();
+      D.n_ksg = Q.ksg.size();
+      D.n_kss = Q.kss.size();
+      D.max_group_bytes = 0;
+      // the K layout of class k of group gi, from whichever record holds it
+      auto gks = [&](size_t gi, int k) {
+         const GroupRec &g = Q.groups[gi];
+         return (g.ksmode & KS_SETLEVEL) ? ks_from_set(Q.kss[g.ks], tp, k) : Q.ksg[(size_t)g.ks + k];
+      };
       D.class_used.assign(tp.nclass, 0);
-      for (size_t gi = 0; gi < D.n_groups; gi++)
+      for (size_t gi = 0; gi < D.n_groups; gi++) {
+         if (Q.groups[gi].ksmode & KS_SETLEVEL) {
+            for (int k = 0; k < tp.nclass; k++) D.class_used[k] = 1;
+            continue;
+         }
          for (int k = 0; k < tp.nclass; k++)
-            if (ks_count(Q.ksg[gi * tp.nclass + k].flags)) D.class_used[k] = 1;
-      // chunks of whole groups whose primitive blocks fit the work buffer
+            if (ks_count(gks(gi, k).flags)) D.class_used[k] = 1;
+      }
+      // chunks of whole groups whose primitive blocks fit the work buffer; a
+      // set quartet larger than the buffer gets a chunk of its own, for which
+      // the buffer grows (ensure_work)
       Chunk c{};
       size_t bytes = 0;
       for (size_t gi = 0; gi < D.n_groups; gi++) {
@@ -766,7 +895,7 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
          size_t gp = 0;
          for (uint32_t r = g.row0; r < g.row1; r++) gp += Q.rows[r].n_prm;
          const size_t gb = gp * sizeof(double) * (size_t)tp.ncp;
-         if (gb > work) fatal("one set quartet needs more device memory than set_max_mem allows (MiB)", (long long)(gb >> 20));
+         D.max_group_bytes = std::max(D.max_group_bytes, gb);
          if (bytes + gb > work && c.g1 > c.g0) {
             D.chunks.push_back(c);
             c = Chunk{};
@@ -786,7 +915,7 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
          bytes += gb;
          // algorithmic FLOPs of the quartet kernel: contraction, HRR, SPH, KS
          for (int k = 0; k < tp.nclass; k++) {
-            const double nq = ks_count(Q.ksg[gi * tp.nclass + k].flags);
+            const double nq = ks_count(gks(gi, k).flags);
             const ClassPlan &cp = ti.cplans[k];
             const double f = nq * (2.0 * (double)gp * cp.ncp + (double)(g.row1 - g.row0) * cp.hrr_flops + cp.sph_flops +
                                    9.0 * cp.Ns);
@@ -799,9 +928,9 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
    }
 }
 
-void Engine::ensure_work(size_t ncp_max, size_t max_prm_chunk)
+void Engine::ensure_work(size_t ncp_max, size_t max_prm_chunk, size_t max_group_bytes)
 {
-   const size_t want = budget_bytes_ / 2;
+   const size_t want = std::max(budget_bytes_ / 2, max_group_bytes);
    if (!d_work_ || work_bytes_ != want) {
       synchronize();
       if (d_work_) HIPCHECK(hipFree(d_work_));
@@ -895,6 +1024,7 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
    b.prm_row = td.prm_row;
    b.prm_ip = td.prm_ip;
    b.ksg = td.ksg;
+   b.kss = td.kss;
    b.ksx = td.ksx;
    b.data = d_data_;
    b.ac0 = d_work_;
@@ -919,7 +1049,7 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
       kt.K[s] = d_K_[s];
    }
 
-   HIPCHECK(hipMemsetAsync(d_fixcnt_, 0, sizeof(uint32_t), st));
+   if (launch_zero_u32(d_fixcnt_, 1, st)) fatal("counter reset launch", 0);
    ev_begin(0, st);
    ev_key_ = tp.L;
    int rc = launch_fm_r(tp, t, b, st);
@@ -973,16 +1103,20 @@ void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
    if (max_index >= FP_size_) fatal("add_qrtt offsets reach beyond the P / K size of set_P", (long long)max_index);
    ensure_tables();
    upload_data();
-   size_t ncp_max = 0, max_prm = 0, scr = 0;
+   size_t ncp_max = 0, max_prm = 0, scr = 0, max_gb = 0;
    for (auto &D : devs) {
       if (D.chunks.empty()) continue;
       const TypeInfo &ti = types_[D.type];
+      if (truncated_ && ti.plan.L > nder_)
+         fatal("the truncated-Coulomb table has fewer derivative orders than this quartet's total angular momentum",
+               ti.plan.L);
       ncp_max = std::max(ncp_max, (size_t)ti.plan.ncp);
+      max_gb = std::max(max_gb, D.max_group_bytes);
       for (auto &c : D.chunks) max_prm = std::max(max_prm, (size_t)(c.p1 - c.p0));
       scr = std::max(scr, (size_t)ti.cq_gws * SCRATCH_GRID * GQS_PARTS);
    }
    if (ncp_max == 0) return;
-   ensure_work(ncp_max, max_prm);
+   ensure_work(ncp_max, max_prm, max_gb);
    if (scr > scratch_cap_) {
       synchronize();
       if (d_scratch_) HIPCHECK(hipFree(d_scratch_));

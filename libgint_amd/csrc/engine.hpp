@@ -52,6 +52,7 @@ struct TypeQueue {
    std::vector<uint32_t> prm_ip, prm_row;
    std::vector<GroupRec> groups;
    std::vector<KsGroup> ksg;
+   std::vector<KsSet> kss;
    std::vector<KsRec> ksx;
    bool empty() const { return groups.empty(); }
    void release();
@@ -69,9 +70,11 @@ struct TypeDev {
    GroupRec *groups = nullptr;
    uint32_t *prm_ip = nullptr, *prm_row = nullptr;
    KsGroup *ksg = nullptr;
+   KsSet *kss = nullptr;
    KsRec *ksx = nullptr;
-   size_t cap[6] = {0, 0, 0, 0, 0, 0};   // allocated elements of each array
-   size_t n_rows = 0, n_groups = 0, n_prm = 0, n_ksx = 0;
+   size_t cap[7] = {0, 0, 0, 0, 0, 0, 0};   // allocated elements of each array
+   size_t n_rows = 0, n_groups = 0, n_prm = 0, n_ksx = 0, n_ksg = 0, n_kss = 0;
+   size_t max_group_bytes = 0;            // largest set quartet's primitive blocks
    std::vector<Chunk> chunks;
    std::vector<char> class_used;
    void release();
@@ -123,6 +126,9 @@ class Engine {
                  int ld_ac, int ld_ad, int ld_bc, int ld_bd, unsigned off_ac, unsigned off_ad, unsigned off_bc,
                  unsigned off_bd, bool Tac, bool Tad, bool Tbc, bool Tbd);
    void add_set();
+   // extension: every add_qrt / add_qrtt call of the current set quartet at
+   // once, for K blocks laid out per pair of sets (KsSet)
+   void add_set_blocks(double symm_fac, const int ld[4], const unsigned off[4], const bool T[4]);
    void dispatch_all();
    void synchronize();
 
@@ -139,7 +145,7 @@ class Engine {
    void plan_execute();
    void plan_clear();
    void plan_info(uint64_t out[6]);
-   void queue_info(uint64_t out[6]);
+   void queue_info(uint64_t out[8]);
    int np_of(int i) const { return i < (int)sets_.size() ? sets_[i].np : 0; }
    const double *exponents_of(int i) const { return &store_.data()[sets_[i].Z]; }
 
@@ -147,10 +153,11 @@ class Engine {
    int type_of(int i, int j, int k, int l);
    int set_sig(int i);
    KsGroup compress(std::vector<KsRec> &calls, const TypePlan &tp, int k, std::vector<KsRec> &ksx);
+   bool set_level(const KsGroup *g, const TypePlan &tp, KsSet &s) const;
    void upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool release_host);
    void execute(std::vector<TypeDev> &devs, uint64_t max_index);
    void run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c);
-   void ensure_work(size_t ncp_max, size_t max_prm_chunk);
+   void ensure_work(size_t ncp_max, size_t max_prm_chunk, size_t max_group_bytes);
    void ensure_tables();
    void upload_data();
    void harvest_events();
@@ -173,6 +180,7 @@ class Engine {
    double R_cut_ = 0.0;
    std::vector<double> C0_;
    int ld_C0_ = 0;
+   int nder_ = -1;   // highest derivative order of the C0 table
    double hf_fac_ = 1.0;
    bool tables_dirty_ = true;
 
@@ -193,6 +201,8 @@ class Engine {
    std::array<int, 4> cur_sets_ = {-1, -1, -1, -1};
    std::vector<KsRec> pend_[NCLASS];
    std::vector<int> touched_;
+   KsSet blocks_{};            // add_set_blocks of the open set quartet
+   bool have_blocks_ = false;
    size_t pend_bytes_ = 0;
    uint64_t max_index_ = 0;   // highest P / K element the pending calls reach
    uint64_t plan_max_index_ = 0;
@@ -216,7 +226,8 @@ class Engine {
    double *d_scratch_ = nullptr;
    size_t scratch_cap_ = 0;
    unsigned long long *d_stats_ = nullptr;   // [0..2] kernel statistics, [3] unused
-   uint32_t *d_fixcnt_ = nullptr;             // [0] entries of the chunk, [1] overflow (sticky)
+   uint32_t *d_fixcnt_ = nullptr;             // [0] entries of the chunk, [1] slots beyond the list (sticky)
+   uint64_t fix_overflow_ = 0;                // such slots seen so far (kept by every class)
    bool stats_on_ = false;
    uint64_t n_slots_ = 0, n_prmc_ = 0, n_q_ = 0, n_rows_ = 0;
 

@@ -510,7 +510,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                         b.fix[2 * idx] = p;
                         b.fix[2 * idx + 1] = (uint32_t)n3 | (mask << 8);
                      } else {
-                        atomicOr(&b.fix_count[1], 1u);
+                        // list full: every class keeps the slot (its G_0 is
+                        // within rounding of the 1e-12 threshold)
+                        atomicAdd(&b.fix_count[1], 1u);
+                        status = 1;
+                        n_any++;
                      }
                   } else {
                      n_any++;
@@ -825,10 +829,10 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
    const double symm[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
    const uint32_t nfix = min(b.fix_count[0], b.fix_cap);
    const int gate = tp.cls_gate[cls_k];
-   const KsGroup ks = b.ksg[(size_t)gi * tp.nclass + cls_k];
+   const GroupRec &g = b.groups[gi];
+   const KsGroup ks = group_ks(b, tp, g, cls_k);
    const int nq = (int)(ks.flags >> 16);
    if (nq == 0) return;
-   const GroupRec &g = b.groups[gi];
    const bool expl = (ks.flags & KS_EXPLICIT) != 0;
    const bool reduce = y.reduce && !expl && nq == nq_all;
    for (int l0 = 0; l0 < nq; l0 += y.lb) {
@@ -1134,8 +1138,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
    for (int i = tid; i < 588; i += NT) ((signed char *)(tabs + CQ_TAB_PLUS))[i] = c_cart.plus[i / 84][(i / 3) % 28][i % 3];
    __syncthreads();
    for (uint32_t gi = b.g0 + blockIdx.x; gi < b.g1; gi += gridDim.x) {
-      const KsGroup *ksrow = b.ksg + (size_t)gi * tp.nclass;
       const GroupRec &g = b.groups[gi];
+      const bool setlevel = (g.ksmode & KS_SETLEVEL) != 0;
       // the set quartet's contraction coefficients, shell by shell
       for (int tc = 0; tc < 4; tc++) {
          const int npt = ((g.np4 >> (4 * tc)) & 15) + 1;
@@ -1148,7 +1152,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
          }
       }
       for (int k = 0; k < tp.nclass; k++) {
-         if ((ksrow[k].flags >> 16) == 0 || lays[k].skip || lays[k].part != part) continue;
+         if (lays[k].skip || lays[k].part != part) continue;
+         if (!setlevel && ks_count(b.ksg[(size_t)g.ks + k].flags) == 0) continue;
          const CqLayout &y = lays[k];
          __syncthreads();
          {
@@ -1304,6 +1309,19 @@ int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lay
    else
       hipLaunchKernelGGL(k_gqs<128>, dim3(grid), dim3(128), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
                          b.tp, t, b, kt, gws, gws_stride, cls_bytes, part);
+   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void k_zero_u32(uint32_t *p, int n)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) p[i] = 0u;
+}
+
+int launch_zero_u32(uint32_t *p, int n, void *stream)
+{
+   if (n <= 0) return 0;
+   hipLaunchKernelGGL(k_zero_u32, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, p, n);
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -235,6 +235,14 @@ void make_type_plan(TypePlan &tp)
          co += p.nl[t][s2] * MAXNP;
       }
    p.nco_tot = co;
+   for (int t = 0; t < 4; t++) {
+      int f = 0;
+      for (int s2 = 0; s2 < p.nsh[t]; s2++) {
+         p.fun0[t][s2] = f;
+         f += p.nl[t][s2] * nsph(p.l[t][s2]);
+      }
+      p.nfun[t] = f;
+   }
    p.nL = 0;
    for (int L : Lset) p.Ls[p.nL++] = L;
    for (k = 0; k < p.nclass; k++) {

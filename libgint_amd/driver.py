@@ -27,7 +27,8 @@ class Params(C.Structure):
 
 
 class AbiTable(C.Structure):
-    _fields_ = [(n, _vp) for n in ("add_prm", "add_shell", "add_qrt", "add_qrtt", "add_set", "set_Atom", "set_Atom_L")]
+    _fields_ = [(n, _vp) for n in ("add_prm", "add_shell", "add_qrt", "add_qrtt", "add_set", "set_Atom", "set_Atom_L",
+                                   "add_set_blocks")]
 
 
 # Workloads.  Config 0 has one cell; its cell edge (100 A) only has to make
@@ -62,7 +63,8 @@ def drv():
             "gdrv_stream_record": (_vp, [_vp, C.c_int]),
             "gdrv_stream_info": (None, [_vp, _i64p]),
             "gdrv_stream_free": (None, [_vp]),
-            "gdrv_stream_replay": (None, [_vp, _vp, C.POINTER(AbiTable), _vp, C.c_int64, C.c_int64, C.c_int64]),
+            "gdrv_stream_replay": (None, [_vp, _vp, C.POINTER(AbiTable), _vp, C.c_int64, C.c_int64, C.c_int64,
+                                          C.c_int]),
             "gdrv_symmetrize": (None, [_vp, _dp]),
             "gdrv_dense": (None, [_vp, _dp, _dp]),
         }
@@ -79,9 +81,12 @@ def _ptr(a):
 
 
 def abi_table(lib, prefix="libgint_"):
+    """the call-protocol entry points of `lib` (product or oracle); the
+    set-level extension is libgint_amd_add_set_blocks / oracle_add_set_blocks"""
     t = AbiTable()
     for name, _ in AbiTable._fields_:
-        setattr(t, name, C.cast(getattr(lib, prefix + name), _vp).value)
+        full = ("libgint_amd_" + name) if (name == "add_set_blocks" and prefix == "libgint_") else prefix + name
+        setattr(t, name, C.cast(getattr(lib, full), _vp).value)
     return t
 
 
@@ -195,10 +200,13 @@ class Stream:
         except Exception:
             pass
 
-    def replay(self, table, handle, begin=0, end=None, stride=1):
+    def replay(self, table, handle, begin=0, end=None, stride=1, blocks=False):
+        """set quartets [begin, end) with stride; blocks=True sends each set
+        quartet's K layout as one add_set_blocks call instead of its
+        add_qrt / add_qrtt calls"""
         if end is None:
             end = self.n_sets
-        drv().gdrv_stream_replay(self.system.h, self.h, C.byref(table), handle, begin, end, stride)
+        drv().gdrv_stream_replay(self.system.h, self.h, C.byref(table), handle, begin, end, stride, 1 if blocks else 0)
 
 
 def setup_handle(lib, prefix, system, hf_fac=1.0, max_mem_mib=4096, truncated=True, P=None, Pb=None,
@@ -236,7 +244,7 @@ def setup_handle(lib, prefix, system, hf_fac=1.0, max_mem_mib=4096, truncated=Tr
     return h, table
 
 
-def kbuild(lib, prefix, system, stream, begin=0, end=None, stride=1, K0=None, plan=False, **kw):
+def kbuild(lib, prefix, system, stream, begin=0, end=None, stride=1, K0=None, plan=False, blocks=False, **kw):
     """one K-build of (a sample of) the stream; returns K in block layout
     (as accumulated by the library, before symmetrisation) and the handle.
     With Pb in kw the build is spin-polarised and K is (Ka, Kb).  With
@@ -250,11 +258,11 @@ def kbuild(lib, prefix, system, stream, begin=0, end=None, stride=1, K0=None, pl
             getattr(lib, prefix + "set_K")(h, _ptr(K0), system.P_size)
     if plan:
         lib.libgint_amd_plan_begin(h)
-        stream.replay(table, h, begin, end, stride)
+        stream.replay(table, h, begin, end, stride, blocks)
         lib.libgint_amd_plan_end(h)
         lib.libgint_amd_plan_execute(h)
     else:
-        stream.replay(table, h, begin, end, stride)
+        stream.replay(table, h, begin, end, stride, blocks)
     if polar:
         Ka = np.zeros(system.P_size)
         Kb = np.zeros(system.P_size)

@@ -33,6 +33,7 @@ SIG = {
     "oracle_add_qrt": (None, [_vp] + [C.c_int] * 8),
     "oracle_add_qrtt": (None, [_vp, C.c_double] + [C.c_int] * 12 + [C.c_uint] * 4 + [C.c_bool] * 4),
     "oracle_add_set": (None, [_vp]),
+    "oracle_add_set_blocks": (None, [_vp, C.c_double] + [C.c_int] * 4 + [C.c_uint] * 4 + [C.c_bool] * 4),
     "oracle_dispatch": (None, [_vp]),
     "oracle_norm_psi": (C.c_double, [_dp, _dp, C.c_int, C.c_int]),
     "oracle_stats": (None, [_vp, C.POINTER(C.c_uint64)]),

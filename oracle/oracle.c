@@ -257,6 +257,7 @@ typedef struct {
    double *P[2], *K[2];
    uint32_t *prm_tmp;
    int nprm_tmp, capprm;
+   int cur[4], have_cur; /* sets of the open set quartet (rows with primitives) */
    OClass cls[NCLS];
    uint64_t n_slots, n_surv, n_prmq, n_qrtt, checksum;
 } Oracle;
@@ -465,6 +466,8 @@ void oracle_add_shell(void *h, int i, int j, int k, int l, int n1, int n2)
    Oracle *o = (Oracle *)h;
    if (o->nprm_tmp == 0) return;
    OSet *S[4] = {&o->sets[i], &o->sets[j], &o->sets[k], &o->sets[l]};
+   o->cur[0] = i; o->cur[1] = j; o->cur[2] = k; o->cur[3] = l;
+   o->have_cur = 1;
    for (int sa = 0; sa < S[0]->nsub; sa++)
       for (int sb = 0; sb < S[1]->nsub; sb++)
          for (int sc = 0; sc < S[2]->nsub; sc++)
@@ -524,6 +527,48 @@ void oracle_add_qrtt(void *h, double symm_fac, int la, int lb, int lc, int ld,
    e->T[0] = Tac; e->T[1] = Tad; e->T[2] = Tbc; e->T[3] = Tbd;
    C->used = 1;
    o->n_qrtt++;
+}
+
+/* every add_qrt / add_qrtt call of the open set quartet for K blocks laid
+   out per pair of sets (the product's libgint_amd_add_set_blocks): element
+   (f,g) of the (x,y) pair block is at off + (T ? f ld + g : g ld + f), the
+   functions of a set shell by shell, contraction by contraction, 2l+1 each */
+void oracle_add_set_blocks(void *h, double symm_fac, int ld_ac, int ld_ad, int ld_bc, int ld_bd,
+                           unsigned off_ac, unsigned off_ad, unsigned off_bc, unsigned off_bd,
+                           bool Tac, bool Tad, bool Tbc, bool Tbd)
+{
+   Oracle *o = (Oracle *)h;
+   if (!o->have_cur) return;
+   const OSet *S[4] = {&o->sets[o->cur[0]], &o->sets[o->cur[1]], &o->sets[o->cur[2]], &o->sets[o->cur[3]]};
+   int f0[4][8];
+   for (int t = 0; t < 4; t++) {
+      int f = 0;
+      for (int u = 0; u < S[t]->nsub; u++) { f0[t][u] = f; f += S[t]->nl[u] * (2 * S[t]->l[u] + 1); }
+   }
+   const unsigned base[4] = {off_ac, off_ad, off_bc, off_bd};
+   const int ld[4] = {ld_ac, ld_ad, ld_bc, ld_bd};
+   const bool T[4] = {Tac, Tad, Tbc, Tbd};
+   for (int sa = 0; sa < S[0]->nsub; sa++)
+      for (int sb = 0; sb < S[1]->nsub; sb++)
+         for (int sc = 0; sc < S[2]->nsub; sc++)
+            for (int sd = 0; sd < S[3]->nsub; sd++) {
+               const int la = S[0]->l[sa], lb = S[1]->l[sb], lc = S[2]->l[sc], ld_ = S[3]->l[sd];
+               const int nla = S[0]->nl[sa], nlb = S[1]->nl[sb], nlc = S[2]->nl[sc], nld = S[3]->nl[sd];
+               oracle_add_qrt(h, la, lb, lc, ld_, nla, nlb, nlc, nld);
+               for (int a = 0; a < nla; a++)
+                  for (int b = 0; b < nlb; b++)
+                     for (int c = 0; c < nlc; c++)
+                        for (int d = 0; d < nld; d++) {
+                           const long fa = f0[0][sa] + a * (2 * la + 1), fb = f0[1][sb] + b * (2 * lb + 1);
+                           const long fc = f0[2][sc] + c * (2 * lc + 1), fd = f0[3][sd] + d * (2 * ld_ + 1);
+                           const long rr[4] = {fa, fa, fb, fb}, cc[4] = {fc, fd, fc, fd};
+                           unsigned off[4];
+                           for (int q = 0; q < 4; q++)
+                              off[q] = base[q] + (unsigned)(T[q] ? rr[q] * ld[q] + cc[q] : cc[q] * ld[q] + rr[q]);
+                           oracle_add_qrtt(h, symm_fac, la, lb, lc, ld_, a, b, c, d, ld[0], ld[1], ld[2], ld[3],
+                                           off[0], off[1], off[2], off[3], T[0], T[1], T[2], T[3]);
+                        }
+            }
 }
 
 /* ------------------------------------------------------------------ */
@@ -1054,6 +1099,7 @@ static void o_ks(const OKs *e, int la, int lb, int lc, int ld, const double *I, 
 void oracle_add_set(void *h)
 {
    Oracle *o = (Oracle *)h;
+   o->have_cur = 0;
    for (int c = 0; c < NCLS; c++) {
       OClass *C = &o->cls[c];
       if (!C->used) continue;

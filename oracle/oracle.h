@@ -45,6 +45,11 @@ void oracle_add_qrtt(void *h, double symm_fac, int la, int lb, int lc, int ld,
                      unsigned off_ac, unsigned off_ad, unsigned off_bc, unsigned off_bd,
                      bool Tac, bool Tad, bool Tbc, bool Tbd);
 void oracle_add_set(void *h);
+/* the add_qrt / add_qrtt calls of the open set quartet for K blocks laid out
+   per pair of sets (the product's libgint_amd_add_set_blocks) */
+void oracle_add_set_blocks(void *h, double symm_fac, int ld_ac, int ld_ad, int ld_bc, int ld_bd,
+                           unsigned off_ac, unsigned off_ad, unsigned off_bc, unsigned off_bd,
+                           bool Tac, bool Tad, bool Tbc, bool Tbd);
 void oracle_dispatch(void *h);
 
 /* normalisation of one contraction, reference src/extern_functions.cpp:95-110 */

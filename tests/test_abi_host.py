@@ -8,6 +8,7 @@ import math
 import os
 import re
 import shutil
+import sys
 import subprocess
 
 import numpy as np
@@ -64,7 +65,7 @@ def test_compute_norm_psi_closed_form():
 
 
 def _queue(lib, h):
-    out = (C.c_uint64 * 6)()
+    out = (C.c_uint64 * 8)()
     lib.libgint_amd_queue_info(h, out)
     return list(out)
 
@@ -160,3 +161,112 @@ def test_synthetic_cell_list_cap():
     r2 = np.rint(np.sum((gf / (edge / 2)) ** 2, axis=1)).astype(int)
     assert np.all(np.diff(r2) >= 0)
     assert np.all(r2[:27] <= 3) and np.all(r2[27:33] == 4)
+
+
+def _recorded_queue(lib, s, st, blocks, stride=1):
+    from libgint_amd import driver as D
+
+    h = lib.libgint_create_handle()
+    lib.libgint_set_max_mem(h, 65536)
+    hm, hi = s.cell()
+    lib.libgint_set_cell(h, s.periodic, D._ptr(hm), D._ptr(hi))
+    g = s.neighs()
+    lib.libgint_set_neighs(h, D._ptr(g), s.Ng)
+    table = D.abi_table(lib)
+    D.drv().gdrv_emit_basis(s.h, C.byref(table), h)
+    lib.libgint_amd_plan_begin(h)   # recording: nothing reaches the device
+    st.replay(table, h, stride=stride, blocks=blocks)
+    q = _queue(lib, h)
+    lib.libgint_amd_destroy_handle(h)
+    return q
+
+
+def test_set_level_records_from_calls_and_blocks():
+    """the driver's CP2K-like block offsets: every set quartet reduces to one
+    32-byte set-level record, whether its K layout arrives as add_qrt /
+    add_qrtt calls or as one libgint_amd_add_set_blocks call"""
+    from libgint_amd import driver as D, load
+
+    lib = load()
+    s = D.System(n_water=2, n_tm=1, box_A=6.0, image_shells=1, np_max=2, np_tm=2, seed=5)
+    st = s.record()
+    q_calls = _recorded_queue(lib, s, st, blocks=False)
+    q_blocks = _recorded_queue(lib, s, st, blocks=True)
+    assert q_calls[1] == st.n_sets and q_calls[6] == st.n_sets, q_calls
+    assert q_calls == q_blocks
+
+
+def test_oracle_blocks_equal_calls():
+    """the oracle expands a set-level record into the same add_qrt / add_qrtt
+    calls as the per-call stream, so both give the same K bit for bit"""
+    from libgint_amd import driver as D
+
+    o = oracle_lib()
+    s = D.System(n_water=1, n_tm=1, box_A=5.0, image_shells=0, np_max=2, np_tm=1, seed=3)
+    st = s.record()
+    Ka, ha = D.kbuild(o, "oracle_", s, st, blocks=False)
+    Kb, hb = D.kbuild(o, "oracle_", s, st, blocks=True)
+    o.oracle_destroy_handle(ha)
+    o.oracle_destroy_handle(hb)
+    assert np.max(np.abs(Ka)) > 0
+    np.testing.assert_array_equal(Ka, Kb)
+
+
+def oracle_lib():
+    from helpers import oracle
+
+    return oracle()
+
+
+def _abort_script(body):
+    """runs `body` in a fresh interpreter (the library aborts on a refused
+    argument); returns (exit code, stderr)"""
+    code = "import sys; sys.path.insert(0, %r)\n" % ROOT + body
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    return r.returncode, r.stderr
+
+
+_C0_SCRIPT = """
+import ctypes as C, numpy as np
+from libgint_amd import load
+from libgint_amd.driver import tcg_table
+lib = load()
+dp = C.POINTER(C.c_double)
+c0, ld = tcg_table(8)            # derivative orders 0..8
+assert ld == 9 * 105
+h = lib.libgint_create_handle()
+lib.libgint_set_Potential_Truncated(h, 11.3, c0.ctypes.data_as(dp), ld, c0.size)
+R = np.zeros(3); Z = np.array([1.0, 0.4]); K = np.array([0.6, 0.5])
+for i, l in enumerate(({L})):
+    lib.libgint_set_Atom(h, i, R.ctypes.data_as(dp), Z.ctypes.data_as(dp), 2)
+    lib.libgint_set_Atom_L(h, i, l, 1, K.ctypes.data_as(dp))
+lib.libgint_amd_plan_begin(h)
+lib.libgint_add_prm(h, 0, 0, 0, 0)
+lib.libgint_add_shell(h, 0, 1, 2, 3, 0, 0)
+print("accepted", flush=True)
+"""
+
+
+def test_c0_table_of_fewer_orders():
+    """a C0 table of 9 derivative orders (e.g. sized for an spd basis) is
+    accepted; quartets up to L = 8 use it, a quartet of L = 9 is refused
+    with a message instead of reading past the table"""
+    rc, err = _abort_script(_C0_SCRIPT.replace("{L}", "2, 2, 2, 2"))
+    assert rc == 0, err
+    rc, err = _abort_script(_C0_SCRIPT.replace("{L}", "3, 2, 2, 2"))
+    assert rc != 0 and "fewer derivative orders" in err, (rc, err[-500:])
+
+
+def test_config4_lattice_is_the_256_shortest():
+    """the large periodic workload uses 256 lattice vectors, the reference's
+    cell limit: every vector with |g|^2 <= 14 (in cell edges) and 5 of |g|^2 = 17"""
+    from libgint_amd import driver as D
+
+    s = D.System(n_water=1, box_A=6.0, image_shells=3, basis=1, np_max=2, seed=1)
+    assert s.Ng == 256
+    g = s.neighs().reshape(-1, 3)
+    edge = s.cell()[0][0]
+    n2 = np.rint(np.sum((g / edge) ** 2, axis=1)).astype(int)
+    assert np.all(np.diff(n2) >= 0) and n2[0] == 0
+    assert np.sum(n2 <= 14) == 251 and np.all(n2[251:] == 17)
+    assert len({tuple(x) for x in np.rint(g / edge).astype(int)}) == 256
