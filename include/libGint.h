@@ -131,6 +131,12 @@ void libgint_amd_plan_info(void *handle, uint64_t out[6]);
    whose calls reduce to one set-level record (libgint_amd_add_set_blocks),
    [7] bytes of the largest set quartet's primitive blocks */
 void libgint_amd_queue_info(void *handle, uint64_t out[8]);
+/* the same per quartet type with recorded work, 8 numbers each: total
+   angular momentum L, primitive-block doubles ncp, classes, set quartets,
+   rows, primitive quartets, contracted quartets per set quartet, shells per
+   set (4-bit fields); fills at most max_types and returns the number of
+   types (host only, no GPU call) */
+int libgint_amd_queue_types(void *handle, uint64_t *out, int max_types);
 /* Every add_qrt / add_qrtt call of the current set quartet (i,j,k,l) at
    once, for a caller whose P / K blocks are laid out per pair of sets, as
    CP2K's block matrices are: element (f, g) of the (a,c) pair block, f a

@@ -62,6 +62,7 @@ ABI = {
     "libgint_amd_plan_clear": (None, [_vp]),
     "libgint_amd_plan_info": (None, [_vp, C.POINTER(C.c_uint64)]),
     "libgint_amd_queue_info": (None, [_vp, C.POINTER(C.c_uint64)]),
+    "libgint_amd_queue_types": (_i, [_vp, C.POINTER(C.c_uint64), _i]),
     "libgint_amd_add_set_blocks": (None, [_vp, C.c_double] + [_i] * 4 + [_u] * 4 + [_b] * 4),
     "libgint_amd_math_selftest": (_i, [_dp, _dp, _i, _dp]),
     "libgint_amd_math_host": (None, [_dp, _dp, _i, _dp]),

@@ -103,7 +103,7 @@ def build_oracle(verbose=False):
     out = os.path.join(ROOT, "oracle", "_build")
     os.makedirs(out, exist_ok=True)
     so = os.path.join(out, "liboracle.so")
-    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c99", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-std=c99", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
           "-o", so, os.path.join(ROOT, "oracle", "oracle.c"), "-lm"], verbose)
     return so
 

@@ -129,6 +129,7 @@ void libgint_amd_plan_execute(void *h) { E(h)->plan_execute(); }
 void libgint_amd_plan_clear(void *h) { E(h)->plan_clear(); }
 void libgint_amd_plan_info(void *h, uint64_t out[6]) { E(h)->plan_info(out); }
 void libgint_amd_queue_info(void *h, uint64_t out[8]) { E(h)->queue_info(out); }
+int libgint_amd_queue_types(void *h, uint64_t *out, int max_types) { return E(h)->queue_types(out, max_types); }
 int libgint_amd_math_selftest(const double *x, const double *y, int n, double *out)
 {
    return gint::device_math_selftest(x, y, n, out);

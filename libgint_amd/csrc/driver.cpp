@@ -110,6 +110,7 @@ struct GdrvParams {
    double cell_A;      // cell edge; 0 -> box_A (a larger cell makes the minimum image the identity)
    int np_tm;          // primitives of the spdf centres (16 in the preset workloads; fewer for tests)
    int max_cells;      // > 0: keep only the first max_cells lattice vectors (nearest first)
+   int ks_mix;         // 1: vary the order of the add_qrtt calls so that K layouts of every kind occur
 };
 
 struct GdrvSystem {
@@ -497,9 +498,17 @@ void block_of(const GdrvSystem &S, int x, int fx, int y, int fy, unsigned *off, 
    }
 }
 
-void emit_set(const GdrvSystem &S, const GdrvAbi *abi, void *h, int i, int j, int k, int l, int code)
+// ks_mix: set quartet number `nset` sends its calls in the natural order
+// (a set-level layout), or with each class's calls reversed (offsets still
+// affine in the contraction indices, but not the set-level layout), or with
+// the first two calls of each class swapped (not affine: explicit records).
+// The library pairs a call with the contraction of its position, so every
+// order is a valid, different stream.
+void emit_set(const GdrvSystem &S, const GdrvAbi *abi, void *h, int i, int j, int k, int l, int code,
+              int64_t nset = 0)
 {
    static const double fac[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
+   const int mix = S.p.ks_mix ? (int)(nset % 3) : 0;
    const Basis *B[4] = {&S.kinds[S.atoms[i].kind], &S.kinds[S.atoms[j].kind], &S.kinds[S.atoms[k].kind],
                         &S.kinds[S.atoms[l].kind]};
    for (size_t sa = 0; sa < B[0]->sh.size(); sa++)
@@ -508,22 +517,25 @@ void emit_set(const GdrvSystem &S, const GdrvAbi *abi, void *h, int i, int j, in
             for (size_t sd = 0; sd < B[3]->sh.size(); sd++) {
                const Shell &A = B[0]->sh[sa], &Bb = B[1]->sh[sb], &C = B[2]->sh[sc], &D = B[3]->sh[sd];
                abi->add_qrt(h, A.l, Bb.l, C.l, D.l, A.nl, Bb.nl, C.nl, D.nl);
-               for (int a = 0; a < A.nl; a++)
-                  for (int b = 0; b < Bb.nl; b++)
-                     for (int c = 0; c < C.nl; c++)
-                        for (int d = 0; d < D.nl; d++) {
-                           const int fa = fun_in_atom(*B[0], (int)sa, a), fb = fun_in_atom(*B[1], (int)sb, b);
-                           const int fc = fun_in_atom(*B[2], (int)sc, c), fd = fun_in_atom(*B[3], (int)sd, d);
-                           unsigned oac, oad, obc, obd;
-                           int lac, lad, lbc, lbd;
-                           bool Tac, Tad, Tbc, Tbd;
-                           block_of(S, i, fa, k, fc, &oac, &lac, &Tac);
-                           block_of(S, i, fa, l, fd, &oad, &lad, &Tad);
-                           block_of(S, j, fb, k, fc, &obc, &lbc, &Tbc);
-                           block_of(S, j, fb, l, fd, &obd, &lbd, &Tbd);
-                           abi->add_qrtt(h, fac[code], A.l, Bb.l, C.l, D.l, a, b, c, d, lac, lad, lbc, lbd, oac, oad,
-                                         obc, obd, Tac, Tad, Tbc, Tbd);
-                        }
+               const int nq = A.nl * Bb.nl * C.nl * D.nl;
+               for (int n = 0; n < nq; n++) {
+                  int m = n;
+                  if (mix == 1) m = nq - 1 - n;
+                  else if (mix == 2 && nq >= 3 && n < 2) m = 1 - n;
+                  const int a = m / (Bb.nl * C.nl * D.nl), b = (m / (C.nl * D.nl)) % Bb.nl, c = (m / D.nl) % C.nl,
+                            d = m % D.nl;
+                  const int fa = fun_in_atom(*B[0], (int)sa, a), fb = fun_in_atom(*B[1], (int)sb, b);
+                  const int fc = fun_in_atom(*B[2], (int)sc, c), fd = fun_in_atom(*B[3], (int)sd, d);
+                  unsigned oac, oad, obc, obd;
+                  int lac, lad, lbc, lbd;
+                  bool Tac, Tad, Tbc, Tbd;
+                  block_of(S, i, fa, k, fc, &oac, &lac, &Tac);
+                  block_of(S, i, fa, l, fd, &oad, &lad, &Tad);
+                  block_of(S, j, fb, k, fc, &obc, &lbc, &Tbc);
+                  block_of(S, j, fb, l, fd, &obd, &lbd, &Tbd);
+                  abi->add_qrtt(h, fac[code], A.l, Bb.l, C.l, D.l, a, b, c, d, lac, lad, lbc, lbd, oac, oad, obc, obd,
+                                Tac, Tad, Tbc, Tbd);
+               }
             }
    abi->add_set(h);
 }
@@ -717,7 +729,7 @@ void gdrv_stream_replay(const GdrvSystem *S, const GdrvStream *st, const GdrvAbi
             if (blocks)
                emit_set_blocks(*S, abi, h, ij & 0xffff, ij >> 16, kl & 0xffff, kl >> 16, (int)(w & 7));
             else
-               emit_set(*S, abi, h, ij & 0xffff, ij >> 16, kl & 0xffff, kl >> 16, (int)(w & 7));
+               emit_set(*S, abi, h, ij & 0xffff, ij >> 16, kl & 0xffff, kl >> 16, (int)(w & 7), s);
             break;
          }
       }

@@ -746,6 +746,31 @@ void Engine::queue_info(uint64_t out[8])
    }
 }
 
+int Engine::queue_types(uint64_t *out, int max_types)
+{
+   int n = 0;
+   for (size_t ty = 0; ty < q_.size(); ty++) {
+      const TypeQueue &Q = q_[ty];
+      if (Q.empty()) continue;
+      if (n < max_types) {
+         const TypePlan &tp = types_[ty].plan;
+         uint64_t *o = out + 8 * (size_t)n;
+         o[0] = (uint64_t)tp.L;
+         o[1] = (uint64_t)tp.ncp;
+         o[2] = (uint64_t)tp.nclass;
+         o[3] = Q.groups.size();
+         o[4] = Q.rows.size();
+         o[5] = Q.prm_ip.size();
+         uint64_t nq = 0;
+         for (int k = 0; k < tp.nclass; k++) nq += (uint64_t)tp.cls_nl[k];
+         o[6] = nq;   // contracted quartets per set quartet
+         o[7] = (uint64_t)(tp.nsh[0] | tp.nsh[1] << 4 | tp.nsh[2] << 8 | tp.nsh[3] << 12);
+      }
+      n++;
+   }
+   return n;
+}
+
 void Engine::plan_info(uint64_t out[6])
 {
    uint64_t bytes = 0, chunks = 0, groups = 0, rows = 0, prm = 0, nt = 0;

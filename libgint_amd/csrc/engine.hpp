@@ -146,6 +146,7 @@ class Engine {
    void plan_clear();
    void plan_info(uint64_t out[6]);
    void queue_info(uint64_t out[8]);
+   int queue_types(uint64_t *out, int max_types);
    int np_of(int i) const { return i < (int)sets_.size() ? sets_[i].np : 0; }
    const double *exponents_of(int i) const { return &store_.data()[sets_[i].Z]; }
 

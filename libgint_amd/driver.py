@@ -22,7 +22,7 @@ class Params(C.Structure):
         ("n_water", C.c_int), ("n_tm", C.c_int), ("box_A", C.c_double), ("image_shells", C.c_int),
         ("basis", C.c_int), ("np_min", C.c_int), ("np_max", C.c_int), ("seed", C.c_uint64),
         ("rcut_A", C.c_double), ("eps", C.c_double), ("min_oo_A", C.c_double), ("symmetric", C.c_int),
-        ("cell_A", C.c_double), ("np_tm", C.c_int), ("max_cells", C.c_int),
+        ("cell_A", C.c_double), ("np_tm", C.c_int), ("max_cells", C.c_int), ("ks_mix", C.c_int),
     ]
 
 
@@ -100,7 +100,7 @@ def tcg_table(nder=21):
 class System:
     def __init__(self, config=None, **kw):
         p = dict(n_water=1, n_tm=0, box_A=10.0, image_shells=0, basis=0, np_min=1, np_max=4, seed=0,
-                 rcut_A=6.0, eps=1e-11, min_oo_A=2.4, symmetric=1, cell_A=0.0, np_tm=16, max_cells=0)
+                 rcut_A=6.0, eps=1e-11, min_oo_A=2.4, symmetric=1, cell_A=0.0, np_tm=16, max_cells=0, ks_mix=0)
         if config is not None:
             p.update(CONFIGS[config])
         p.update(kw)

@@ -25,6 +25,7 @@
 #include <math.h>
 #include <float.h>
 #include <assert.h>
+#include <omp.h>
 #include "../libgint_amd/csrc/tcg_patches.h"
 #include "../libgint_amd/csrc/c2s_table.h"
 
@@ -239,6 +240,31 @@ typedef struct {
    int nks, capks;
 } OClass;
 
+/* one class of a closed set quartet waiting to be integrated: its rows (it
+   owns their primitive lists) and its K entries */
+typedef struct {
+   int c;
+   ORow *rows;
+   int nrows;
+   unsigned nQ;
+   OKs *ks;
+   int nks;
+} OPendClass;
+
+typedef struct {
+   OPendClass *cls;
+   int ncls;
+} OPendSet;
+
+/* per-thread counters of the integrated slots */
+typedef struct {
+   uint64_t n_slots, n_surv, n_prmq, checksum;
+} OStats;
+
+/* closed set quartets are integrated in batches of this many, one OpenMP
+   thread per set quartet (round robin), each into its own K */
+#define OPEND_BATCH 2048
+
 typedef struct {
    int truncated;
    double R_cut;
@@ -260,7 +286,11 @@ typedef struct {
    int cur[4], have_cur; /* sets of the open set quartet (rows with primitives) */
    OClass cls[NCLS];
    uint64_t n_slots, n_surv, n_prmq, n_qrtt, checksum;
+   OPendSet *pend;
+   int npend, cappend;
 } Oracle;
+
+static void o_flush(Oracle *o);
 
 void *oracle_create_handle(void)
 {
@@ -281,10 +311,26 @@ static void free_class(OClass *c)
    c->used = 0;
 }
 
+static void free_pending(Oracle *o)
+{
+   for (int i = 0; i < o->npend; i++) {
+      OPendSet *ps = &o->pend[i];
+      for (int c = 0; c < ps->ncls; c++) {
+         for (int r = 0; r < ps->cls[c].nrows; r++) free(ps->cls[c].rows[r].prm);
+         free(ps->cls[c].rows);
+         free(ps->cls[c].ks);
+      }
+      free(ps->cls);
+   }
+   o->npend = 0;
+}
+
 void oracle_destroy_handle(void *h)
 {
    Oracle *o = (Oracle *)h;
    if (!o) return;
+   free_pending(o);
+   free(o->pend);
    for (int c = 0; c < NCLS; c++) {
       free_class(&o->cls[c]);
       free(o->cls[c].rows);
@@ -389,6 +435,7 @@ void oracle_set_Atom_L(void *h, int i, int l, int nl, double *K)
 void oracle_set_P(void *h, double *P, int P_size)
 {
    Oracle *o = (Oracle *)h;
+   o_flush(o);
    o->nspin = 1;
    o->FP_size = (size_t)P_size;
    for (int s = 0; s < 2; s++) { free(o->P[s]); free(o->K[s]); o->P[s] = o->K[s] = NULL; }
@@ -400,6 +447,7 @@ void oracle_set_P(void *h, double *P, int P_size)
 void oracle_set_P_polarized(void *h, double *Pa, double *Pb, int P_size)
 {
    Oracle *o = (Oracle *)h;
+   o_flush(o);
    o->nspin = 2;
    o->FP_size = (size_t)P_size;
    for (int s = 0; s < 2; s++) { free(o->P[s]); free(o->K[s]); }
@@ -415,6 +463,7 @@ void oracle_set_P_polarized(void *h, double *Pa, double *Pb, int P_size)
 void oracle_set_K(void *h, double *K, int K_size)
 {
    Oracle *o = (Oracle *)h;
+   o_flush(o);
    free(o->K[0]);
    o->FP_size = (size_t)K_size;
    o->K[0] = (double *)malloc(sizeof(double) * o->FP_size);
@@ -424,6 +473,7 @@ void oracle_set_K(void *h, double *K, int K_size)
 void oracle_set_K_polarized(void *h, double *Ka, double *Kb, int K_size)
 {
    Oracle *o = (Oracle *)h;
+   o_flush(o);
    o->FP_size = (size_t)K_size;
    for (int s = 0; s < 2; s++) free(o->K[s]);
    o->K[0] = (double *)malloc(sizeof(double) * o->FP_size);
@@ -435,12 +485,14 @@ void oracle_set_K_polarized(void *h, double *Ka, double *Kb, int K_size)
 void oracle_get_K(void *h, double *K)
 {
    Oracle *o = (Oracle *)h;
+   o_flush(o);
    memcpy(K, o->K[0], sizeof(double) * o->FP_size);
 }
 
 void oracle_get_K_polarized(void *h, double *Ka, double *Kb)
 {
    Oracle *o = (Oracle *)h;
+   o_flush(o);
    memcpy(Ka, o->K[0], sizeof(double) * o->FP_size);
    memcpy(Kb, o->K[1], sizeof(double) * o->FP_size);
 }
@@ -788,7 +840,7 @@ static void o_vrr(VWork *w, const double *Fm, const double *pq)
    src/compute_VRR.cpp:349-458 (VRR and the Fm[0] > 1e-12 gate),
    src/compute_ECO.cpp:216-272 (sum over n3), src/compute_ECO.cpp:12-214
    (contraction), src/compute_HRR.cpp:7-225 (HRR) */
-static void o_row(Oracle *o, const ORow *r, int la, int lb, int lc, int ld, double *out)
+static void o_row(const Oracle *o, OStats *st, const ORow *r, int la, int lb, int lc, int ld, double *out)
 {
    cart_tables();
    RowGeo G;
@@ -887,7 +939,7 @@ static void o_row(Oracle *o, const ORow *r, int la, int lb, int lc, int ld, doub
       const double izab = 1. / zab, i2ab = 0.5 * izab, mrab = -1. * rho * izab * i2ab;
       const double izcd = 1. / zcd2, i2cd = 0.5 * izcd, mrcd = -1. * rho * izcd * i2cd;
       const double i2z = 0.5 * inv_z;
-      o->n_prmq++;
+      st->n_prmq++;
 
       memset(AC0, 0, sizeof(double) * (size_t)ncp);
       memset(ACr, 0, sizeof(double) * (size_t)ncp);
@@ -904,11 +956,11 @@ static void o_row(Oracle *o, const ORow *r, int la, int lb, int lc, int ld, doub
          } else {
             o_fgamma0(Lsum, T, Fm, s_ftable(), Kfactor);
          }
-         o->n_slots++;
+         st->n_slots++;
          double *dst = (n3 == 0) ? AC0 : ACr;
          if (Fm[0] > 1.e-12) {
-            o->n_surv++;
-            o->checksum += oracle_slot_key(r->i, r->j, r->k, r->l, r->n1, r->n2, la, lb, lc, ld, ipa, ipb, ipc, ipd, n3);
+            st->n_surv++;
+            st->checksum += oracle_slot_key(r->i, r->j, r->k, r->l, r->n1, r->n2, la, lb, lc, ld, ipa, ipb, ipc, ipd, n3);
             double Qp[3] = {Q[0] + neighs[n3 * 3 + 0], Q[1] + neighs[n3 * 3 + 1], Q[2] + neighs[n3 * 3 + 2]};
             double W[3];
             W[0] = (zab * P[0] + zcd2 * Qp[0]) * inv_z;
@@ -1094,48 +1146,122 @@ static void o_ks(const OKs *e, int la, int lb, int lc, int ld, const double *I, 
    }
 }
 
-/* reference src/libGint.cpp:285-414 (merge of the set) followed by the
-   per-class chain of src/libGint.cpp:705-997 for this set's quartets */
+/* the per-class chain of reference src/libGint.cpp:705-997 for one closed
+   set quartet, into K (one thread's copy) */
+static void o_integrate_set(const Oracle *o, const OPendSet *ps, double *K[2], OStats *st)
+{
+   for (int ci = 0; ci < ps->ncls; ci++) {
+      const OPendClass *C = &ps->cls[ci];
+      const int c = C->c;
+      const int la = c / 64, lb = (c / 16) % 4, lc = (c / 4) % 4, ld = c % 4;
+      const int Nc = NCO(la) * NCO(lb) * NCO(lc) * NCO(ld);
+      const int Ns = (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1) * (2 * ld + 1);
+      if (C->nQ == 0 || !o->P[0]) continue;
+      double *ABCD0 = (double *)calloc((size_t)C->nQ * Nc, sizeof(double));
+      for (int r = 0; r < C->nrows; r++) {
+         const ORow *row = &C->rows[r];
+         const OSet *S[4] = {&o->sets[row->i], &o->sets[row->j], &o->sets[row->k], &o->sets[row->l]};
+         int nlabcd = S[0]->nl[row->sa] * S[1]->nl[row->sb] * S[2]->nl[row->sc] * S[3]->nl[row->sd];
+         if (row->Oq + (unsigned)nlabcd > C->nQ) continue; /* no quartet declared for it */
+         double *out = (double *)malloc(sizeof(double) * (size_t)nlabcd * Nc);
+         o_row(o, st, row, la, lb, lc, ld, out);
+         for (size_t t = 0; t < (size_t)nlabcd * Nc; t++) ABCD0[(size_t)row->Oq * Nc + t] += out[t];
+         free(out);
+      }
+      double *SPH = (double *)malloc(sizeof(double) * (size_t)C->nQ * Ns);
+      double *t1 = (double *)malloc(sizeof(double) * (size_t)Nc * 4);
+      double *t2 = (double *)malloc(sizeof(double) * (size_t)Nc * 4);
+      for (unsigned q = 0; q < C->nQ; q++) o_sph(la, lb, lc, ld, ABCD0 + (size_t)q * Nc, SPH + (size_t)q * Ns, t1, t2);
+      for (int e = 0; e < C->nks && (unsigned)e < C->nQ; e++)
+         for (int s = 0; s < o->nspin; s++)
+            o_ks(&C->ks[e], la, lb, lc, ld, SPH + (size_t)e * Ns, o->P[s], K[s], o->hf_fac);
+      free(t1); free(t2); free(SPH); free(ABCD0);
+   }
+}
+
+/* integrates the pending set quartets: set quartet i on thread i mod T, each
+   thread into a private K, then the private K summed into K in thread order
+   (the result depends on T only through the order of that sum) */
+static void o_flush(Oracle *o)
+{
+   if (o->npend == 0) return;
+   cart_tables();
+   (void)s_ftable();
+   const int T = omp_get_max_threads();
+   const size_t n = o->FP_size;
+   const int nspin = o->nspin > 0 ? o->nspin : 1;
+   double *priv = NULL;
+   if (o->P[0]) priv = (double *)calloc((size_t)T * (size_t)nspin * (n ? n : 1), sizeof(double));
+   OStats *st = (OStats *)calloc((size_t)T, sizeof(OStats));
+#pragma omp parallel num_threads(T)
+   {
+      const int t = omp_get_thread_num();
+      double *Kt[2] = {NULL, NULL};
+      if (priv)
+         for (int s = 0; s < nspin; s++) Kt[s] = priv + ((size_t)t * nspin + s) * n;
+#pragma omp for schedule(static, 1)
+      for (int i = 0; i < o->npend; i++) o_integrate_set(o, &o->pend[i], Kt, &st[t]);
+   }
+   if (priv) {
+      for (int s = 0; s < o->nspin; s++)
+         for (int t = 0; t < T; t++) {
+            const double *Kt = priv + ((size_t)t * nspin + s) * n;
+            for (size_t x = 0; x < n; x++) o->K[s][x] += Kt[x];
+         }
+   }
+   for (int t = 0; t < T; t++) {
+      o->n_slots += st[t].n_slots;
+      o->n_surv += st[t].n_surv;
+      o->n_prmq += st[t].n_prmq;
+      o->checksum += st[t].checksum;
+   }
+   free(st);
+   free(priv);
+   free_pending(o);
+}
+
+/* reference src/libGint.cpp:285-414: the set quartet is closed; its classes
+   are integrated with the next batch */
 void oracle_add_set(void *h)
 {
    Oracle *o = (Oracle *)h;
    o->have_cur = 0;
+   int nused = 0;
+   for (int c = 0; c < NCLS; c++) nused += o->cls[c].used;
+   if (nused == 0) return;
+   if (o->npend == o->cappend) {
+      o->cappend = o->cappend ? 2 * o->cappend : 64;
+      o->pend = (OPendSet *)realloc(o->pend, sizeof(OPendSet) * (size_t)o->cappend);
+   }
+   OPendSet *ps = &o->pend[o->npend++];
+   ps->cls = (OPendClass *)malloc(sizeof(OPendClass) * (size_t)nused);
+   ps->ncls = 0;
    for (int c = 0; c < NCLS; c++) {
       OClass *C = &o->cls[c];
       if (!C->used) continue;
-      const int la = c / 64, lb = (c / 16) % 4, lc = (c / 4) % 4, ld = c % 4;
-      const int Nc = NCO(la) * NCO(lb) * NCO(lc) * NCO(ld);
-      const int Ns = (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1) * (2 * ld + 1);
-      if (C->nQ > 0 && o->P[0]) {
-         double *ABCD0 = (double *)calloc((size_t)C->nQ * Nc, sizeof(double));
-         for (int r = 0; r < C->nrows; r++) {
-            const ORow *row = &C->rows[r];
-            const OSet *S[4] = {&o->sets[row->i], &o->sets[row->j], &o->sets[row->k], &o->sets[row->l]};
-            int nlabcd = S[0]->nl[row->sa] * S[1]->nl[row->sb] * S[2]->nl[row->sc] * S[3]->nl[row->sd];
-            if (row->Oq + (unsigned)nlabcd > C->nQ) continue; /* no quartet declared for it */
-            double *out = (double *)malloc(sizeof(double) * (size_t)nlabcd * Nc);
-            o_row(o, row, la, lb, lc, ld, out);
-            for (size_t t = 0; t < (size_t)nlabcd * Nc; t++) ABCD0[(size_t)row->Oq * Nc + t] += out[t];
-            free(out);
-         }
-         double *SPH = (double *)malloc(sizeof(double) * (size_t)C->nQ * Ns);
-         double *t1 = (double *)malloc(sizeof(double) * (size_t)Nc * 4);
-         double *t2 = (double *)malloc(sizeof(double) * (size_t)Nc * 4);
-         for (unsigned q = 0; q < C->nQ; q++) o_sph(la, lb, lc, ld, ABCD0 + (size_t)q * Nc, SPH + (size_t)q * Ns, t1, t2);
-         for (int e = 0; e < C->nks && (unsigned)e < C->nQ; e++)
-            for (int s = 0; s < o->nspin; s++)
-               o_ks(&C->ks[e], la, lb, lc, ld, SPH + (size_t)e * Ns, o->P[s], o->K[s], o->hf_fac);
-         free(t1); free(t2); free(SPH); free(ABCD0);
-      }
-      free_class(C);
+      OPendClass *pc = &ps->cls[ps->ncls++];
+      pc->c = c;
+      pc->nrows = C->nrows;
+      pc->rows = (ORow *)malloc(sizeof(ORow) * (size_t)(C->nrows ? C->nrows : 1));
+      memcpy(pc->rows, C->rows, sizeof(ORow) * (size_t)C->nrows);   /* the primitive lists move too */
+      pc->nQ = C->nQ;
+      pc->nks = C->nks;
+      pc->ks = (OKs *)malloc(sizeof(OKs) * (size_t)(C->nks ? C->nks : 1));
+      memcpy(pc->ks, C->ks, sizeof(OKs) * (size_t)C->nks);
+      C->nrows = 0;
+      C->nks = 0;
+      C->nQ = 0;
+      C->used = 0;
    }
+   if (o->npend >= OPEND_BATCH) o_flush(o);
 }
 
-void oracle_dispatch(void *h) { (void)h; }
+void oracle_dispatch(void *h) { o_flush((Oracle *)h); }
 
 void oracle_stats(void *h, uint64_t out[5])
 {
    Oracle *o = (Oracle *)h;
+   o_flush(o);
    out[0] = o->n_slots;
    out[1] = o->n_surv;
    out[2] = o->n_prmq;
@@ -1155,5 +1281,6 @@ void oracle_row_cart(void *h, int i, int j, int k, int l, int n1, int n2, int sa
    r.nprm = nprm;
    r.prm = (uint32_t *)prm;
    r.Oq = 0;
-   o_row(o, &r, o->sets[i].l[sa], o->sets[j].l[sb], o->sets[k].l[sc], o->sets[l].l[sd], out);
+   OStats st = {0, 0, 0, 0};
+   o_row(o, &st, &r, o->sets[i].l[sa], o->sets[j].l[sb], o->sets[k].l[sc], o->sets[l].l[sd], out);
 }

@@ -270,3 +270,15 @@ def test_config4_lattice_is_the_256_shortest():
     assert np.all(np.diff(n2) >= 0) and n2[0] == 0
     assert np.sum(n2 <= 14) == 251 and np.all(n2[251:] == 17)
     assert len({tuple(x) for x in np.rint(g / edge).astype(int)}) == 256
+
+
+def test_mixed_layouts_reach_every_record_kind():
+    """ks_mix: the set quartets' calls arrive as set-level layouts, per-class
+    affine layouts and explicit lists, a third each"""
+    from libgint_amd import driver as D, load
+
+    lib = load()
+    s = D.System(n_water=2, box_A=6.0, image_shells=1, np_max=2, seed=5, ks_mix=1)
+    st = s.record()
+    q = _recorded_queue(lib, s, st, blocks=False)
+    assert 0 < q[6] < st.n_sets and q[5] > 0 and q[4] > q[5] + q[6], q

@@ -91,6 +91,19 @@ CASES = [
 ]
 
 
+CASES += [
+    # K layouts of every record kind: a third of the set quartets set-level,
+    # a third per-class affine (calls reversed), a third explicit lists
+    ("water2_pbc27_ksmix", dict(n_water=2, box_A=7.0, image_shells=1, np_max=3, seed=21, ks_mix=1), {}),
+    ("water1_tm1_pbc27_ksmix", dict(n_water=1, n_tm=1, box_A=6.0, image_shells=1, np_max=2, np_tm=2, seed=22,
+                                    ks_mix=1), {}),
+    # the K layout of each set quartet in one libgint_amd_add_set_blocks call
+    ("water2_pbc27_blocks", dict(n_water=2, box_A=7.0, image_shells=1, np_max=3, seed=23), dict(blocks=True)),
+    ("water1_tm1_pbc27_blocks_plan", dict(n_water=1, n_tm=1, box_A=6.0, image_shells=1, np_max=2, np_tm=2, seed=24),
+     dict(blocks=True, plan=True)),
+]
+
+
 @pytest.mark.parametrize("name,sysargs,kw", CASES, ids=[c[0] for c in CASES])
 def test_kbuild_parity(name, sysargs, kw):
     from libgint_amd import driver as D
@@ -114,3 +127,37 @@ def test_polarized_and_set_K():
     Kp, Ko, sp, so = _both(s, st, P=P, Pb=Pb, K0=K0)
     _check(Kp[0], Ko[0], sp, so, "polar_a")
     _check(Kp[1], Ko[1], sp, so, "polar_b")
+
+
+@pytest.mark.parametrize("debug", [0, 1], ids=["direct", "fixup_path"])
+def test_plan_replay_new_densities(debug):
+    """a resident plan recorded once and executed for three densities in turn
+    (set_P zeroes K again and the new P is read): each K against the oracle"""
+    import ctypes as C
+    from libgint_amd import driver as D, load
+
+    lib = load()
+    o = oracle()
+    s = D.System(n_water=2, box_A=7.0, image_shells=1, np_max=3, seed=12)
+    st = s.record()
+    rng = np.random.default_rng(3)
+    P1 = s.P()
+    P2 = P1 * (1.0 + 0.2 * rng.standard_normal(P1.size))
+    h, table = D.setup_handle(lib, "libgint_", s, P=P1, stats=True, debug=debug)
+    lib.libgint_amd_plan_begin(h)
+    st.replay(table, h)
+    lib.libgint_amd_plan_end(h)
+    prev = D.stats(lib, "libgint_", h)
+    for i, P in enumerate((P1, P2, P1)):
+        lib.libgint_set_P(h, D._ptr(P), s.P_size)
+        lib.libgint_amd_plan_execute(h)
+        Kp = np.zeros(s.P_size)
+        lib.libgint_get_K(h, D._ptr(Kp))
+        cur = D.stats(lib, "libgint_", h)
+        step = [cur[1] - prev[1], 0, 0, 0, (cur[4] - prev[4]) % (1 << 64)]
+        prev = cur
+        Ko, ho = D.kbuild(o, "oracle_", s, st, P=P)
+        so = D.stats(o, "oracle_", ho)
+        o.oracle_destroy_handle(ho)
+        _check(Kp, Ko, step, so, "replay%d_debug%d" % (i, debug))
+    lib.libgint_amd_destroy_handle(h)
