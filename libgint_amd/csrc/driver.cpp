@@ -111,6 +111,11 @@ struct GdrvParams {
    int np_tm;          // primitives of the spdf centres (16 in the preset workloads; fewer for tests)
    int max_cells;      // > 0: keep only the first max_cells lattice vectors (nearest first)
    int ks_mix;         // 1: vary the order of the add_qrtt calls so that K layouts of every kind occur
+   double zmin, zmax;  // range of the log-uniform exponent draw (bohr^-2)
+   int shift_atom;     // >= 0: this atom is moved by shift_cells lattice vectors after placement
+   int shift_cells[3];
+   int replicate;      // > 1: the system repeated this many times along x in a cell that many times longer,
+                       // with the density of every copy pair equal to the original pair's
 };
 
 struct GdrvSystem {
@@ -139,12 +144,12 @@ double norm_psi(const double *K, const double *z, int np, int l)
    return std::sqrt(ans);
 }
 
-Basis make_basis(Rng &rng, const std::vector<std::pair<int, int>> &lnl, int np)
+Basis make_basis(Rng &rng, const std::vector<std::pair<int, int>> &lnl, int np, double zmin, double zmax)
 {
    Basis b;
    b.np = np;
    std::vector<double> z(np);
-   for (int i = 0; i < np; i++) z[i] = std::exp(std::log(0.1) + rng.uniform() * (std::log(5e3) - std::log(0.1)));
+   for (int i = 0; i < np; i++) z[i] = std::exp(std::log(zmin) + rng.uniform() * (std::log(zmax) - std::log(zmin)));
    std::sort(z.begin(), z.end(), [](double a, double c) { return a > c; });
    for (int i = 0; i < np; i++) b.Z[i] = z[i];
    b.cmax_p.assign(np, 0.0);
@@ -171,10 +176,106 @@ Basis make_basis(Rng &rng, const std::vector<std::pair<int, int>> &lnl, int np)
    return b;
 }
 
+// lattice vectors of the {-r..r}^3 block of a rectangular cell of edges
+// e[0..2], nearest first by cell indices (stable: ties keep the loop order);
+// r = 3 keeps the 256 shortest, the reference's cell limit (src/define.h:42),
+// which covers |g|^2 <= 14 (251 vectors) and 5 of the 24 at |g|^2 = 17
+void make_neighs(GdrvSystem &S, const double e[3])
+{
+   const GdrvParams &p = S.p;
+   std::vector<std::array<int, 3>> v;
+   const int r = p.image_shells <= 0 ? 0 : std::min(p.image_shells, 3);
+   for (int a = -r; a <= r; a++)
+      for (int b = -r; b <= r; b++)
+         for (int c = -r; c <= r; c++) v.push_back({a, b, c});
+   std::stable_sort(v.begin(), v.end(), [](const std::array<int, 3> &x, const std::array<int, 3> &y) {
+      return x[0] * x[0] + x[1] * x[1] + x[2] * x[2] < y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+   });
+   if (p.image_shells >= 3 && v.size() > 256) v.resize(256);
+   if (p.max_cells > 0 && (size_t)p.max_cells < v.size()) v.resize(p.max_cells);
+   S.neighs.clear();
+   for (auto &g : v)
+      for (int k = 0; k < 3; k++) S.neighs.push_back(g[k] * e[k]);
+}
+
+// function offsets, pair blocks (i <= j, column-major, ld = functions of i)
+// and the P size of the current atom list
+void layout_blocks(GdrvSystem &S)
+{
+   const int na = (int)S.atoms.size();
+   S.fun0.resize(na);
+   S.nbf = 0;
+   for (int a = 0; a < na; a++) {
+      S.fun0[a] = S.nbf;
+      S.nbf += S.kinds[S.atoms[a].kind].nfun;
+   }
+   S.blk.assign((size_t)na * na, 0);
+   size_t off = 0;
+   for (int i = 0; i < na; i++)
+      for (int j = i; j < na; j++) {
+         S.blk[(size_t)i * na + j] = (uint32_t)off;
+         off += (size_t)S.kinds[S.atoms[i].kind].nfun * S.kinds[S.atoms[j].kind].nfun;
+      }
+   S.P_size = off;
+}
+
+void make_pmax(GdrvSystem &S)
+{
+   const int na = (int)S.atoms.size();
+   S.Pmax.assign((size_t)na * na, 0.0);
+   for (int a = 0; a < na; a++)
+      for (int b = a; b < na; b++) {
+         const size_t base = S.blk[(size_t)a * na + b];
+         const size_t n = (size_t)S.kinds[S.atoms[a].kind].nfun * S.kinds[S.atoms[b].kind].nfun;
+         double m = 0;
+         for (size_t t = 0; t < n; t++) m = std::max(m, std::fabs(S.P[base + t]));
+         S.Pmax[(size_t)a * na + b] = S.Pmax[(size_t)b * na + a] = m;
+      }
+}
+
+// the system repeated nx times along x: copy k of atom a is atom k na + a,
+// at R + k edge x; the cell is nx edges long in x; the density block of two
+// copies is the original block of the two atoms (so that, summed over the
+// copies of its second atom, the exchange matrix of the longer cell is the
+// original one)
+void replicate_x(GdrvSystem &S, int nx, double edge)
+{
+   const int na = (int)S.atoms.size();
+   const std::vector<Atom> a0 = S.atoms;
+   const std::vector<uint32_t> blk0 = S.blk;
+   const std::vector<double> P0 = S.P;
+   S.atoms.clear();
+   for (int k = 0; k < nx; k++)
+      for (int a = 0; a < na; a++) {
+         Atom t = a0[a];
+         t.R[0] += k * edge;
+         S.atoms.push_back(t);
+      }
+   S.cell18[0] = nx * edge;
+   S.cell18[9] = 1.0 / (nx * edge);
+   const double e[3] = {nx * edge, S.cell18[4], S.cell18[8]};
+   make_neighs(S, e);
+   layout_blocks(S);
+   const int nn = (int)S.atoms.size();
+   S.P.assign(S.P_size, 0.0);
+   for (int I = 0; I < nn; I++)
+      for (int J = I; J < nn; J++) {
+         const int i = I % na, j = J % na;
+         const int ni = S.kinds[a0[i].kind].nfun, nj = S.kinds[a0[j].kind].nfun;
+         double *B = &S.P[S.blk[(size_t)I * nn + J]];
+         for (int r = 0; r < ni; r++)
+            for (int c = 0; c < nj; c++)
+               B[r + (size_t)c * ni] = i <= j ? P0[blk0[(size_t)i * na + j] + r + (size_t)c * ni]
+                                              : P0[blk0[(size_t)j * na + i] + c + (size_t)r * nj];
+      }
+   make_pmax(S);
+}
+
 void build(GdrvSystem &S)
 {
    const GdrvParams &p = S.p;
    Rng rng(p.seed);
+   const double zmin = p.zmin > 0 ? p.zmin : 0.1, zmax = p.zmax > 0 ? p.zmax : 5e3;
    const double box = p.box_A * BOHR_PER_ANG;
    const double cedge = (p.cell_A > 0 ? p.cell_A : p.box_A) * BOHR_PER_ANG;
    for (int i = 0; i < 9; i++) {
@@ -182,35 +283,20 @@ void build(GdrvSystem &S)
       S.cell18[9 + i] = (i % 4 == 0) ? 1.0 / cedge : 0.0;
    }
    {
-      std::vector<std::array<int, 3>> v;
-      // the {-r..r}^3 block, nearest first (stable: ties keep the loop
-      // order); image_shells 3 keeps the 256 shortest of {-3..3}^3, the
-      // reference's cell limit (src/define.h:42), which covers |g|^2 <= 14
-      // (251 vectors) and 5 of the 24 at |g|^2 = 17
-      const int r = p.image_shells <= 0 ? 0 : std::min(p.image_shells, 3);
-      for (int a = -r; a <= r; a++)
-         for (int b = -r; b <= r; b++)
-            for (int c = -r; c <= r; c++) v.push_back({a, b, c});
-      std::stable_sort(v.begin(), v.end(), [](const std::array<int, 3> &x, const std::array<int, 3> &y) {
-         return x[0] * x[0] + x[1] * x[1] + x[2] * x[2] < y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
-      });
-      if (p.image_shells >= 3 && v.size() > 256) v.resize(256);
-      if (p.max_cells > 0 && (size_t)p.max_cells < v.size()) v.resize(p.max_cells);
-      S.neighs.clear();
-      for (auto &g : v)
-         for (int k = 0; k < 3; k++) S.neighs.push_back(g[k] * cedge);
+      const double e[3] = {cedge, cedge, cedge};
+      make_neighs(S, e);
    }
    if (p.basis == 4) {
-      S.kinds.push_back(make_basis(rng, {{0, 4}, {1, 3}, {2, 2}, {3, 1}}, rng.range(p.np_min, p.np_max)));
-      S.kinds.push_back(make_basis(rng, {{0, 3}, {1, 2}, {2, 1}}, rng.range(p.np_min, p.np_max)));
+      S.kinds.push_back(make_basis(rng, {{0, 4}, {1, 3}, {2, 2}, {3, 1}}, rng.range(p.np_min, p.np_max), zmin, zmax));
+      S.kinds.push_back(make_basis(rng, {{0, 3}, {1, 2}, {2, 1}}, rng.range(p.np_min, p.np_max), zmin, zmax));
    } else if (p.basis == 1) {
-      S.kinds.push_back(make_basis(rng, {{0, 2}, {1, 1}}, rng.range(p.np_min, p.np_max)));
-      S.kinds.push_back(make_basis(rng, {{0, 1}}, rng.range(p.np_min, p.np_max)));
+      S.kinds.push_back(make_basis(rng, {{0, 2}, {1, 1}}, rng.range(p.np_min, p.np_max), zmin, zmax));
+      S.kinds.push_back(make_basis(rng, {{0, 1}}, rng.range(p.np_min, p.np_max), zmin, zmax));
    } else {
-      S.kinds.push_back(make_basis(rng, {{0, 3}, {1, 2}, {2, 1}}, rng.range(p.np_min, p.np_max)));
-      S.kinds.push_back(make_basis(rng, {{0, 2}, {1, 1}}, rng.range(p.np_min, p.np_max)));
+      S.kinds.push_back(make_basis(rng, {{0, 3}, {1, 2}, {2, 1}}, rng.range(p.np_min, p.np_max), zmin, zmax));
+      S.kinds.push_back(make_basis(rng, {{0, 2}, {1, 1}}, rng.range(p.np_min, p.np_max), zmin, zmax));
    }
-   S.kinds.push_back(make_basis(rng, {{0, 2}, {1, 2}, {2, 2}, {3, 4}}, p.np_tm > 0 ? p.np_tm : 16));
+   S.kinds.push_back(make_basis(rng, {{0, 2}, {1, 2}, {2, 2}, {3, 4}}, p.np_tm > 0 ? p.np_tm : 16, zmin, zmax));
 
    const double dOH = 0.96 * BOHR_PER_ANG, ang = 104.5 * M_PI / 180.0;
    const double dmin = p.min_oo_A * BOHR_PER_ANG;
@@ -260,21 +346,11 @@ void build(GdrvSystem &S)
       S.atoms.push_back(T);
    }
 
+   if (p.shift_atom >= 0 && p.shift_atom < (int)S.atoms.size())
+      for (int k = 0; k < 3; k++) S.atoms[p.shift_atom].R[k] += p.shift_cells[k] * cedge;
+
    const int na = (int)S.atoms.size();
-   S.fun0.resize(na);
-   S.nbf = 0;
-   for (int a = 0; a < na; a++) {
-      S.fun0[a] = S.nbf;
-      S.nbf += S.kinds[S.atoms[a].kind].nfun;
-   }
-   S.blk.assign((size_t)na * na, 0);
-   size_t off = 0;
-   for (int i = 0; i < na; i++)
-      for (int j = i; j < na; j++) {
-         S.blk[(size_t)i * na + j] = (uint32_t)off;
-         off += (size_t)S.kinds[S.atoms[i].kind].nfun * S.kinds[S.atoms[j].kind].nfun;
-      }
-   S.P_size = off;
+   layout_blocks(S);
 
    const int nocc = 40, nbf = S.nbf;
    std::vector<double> C((size_t)nbf * nocc);
@@ -304,15 +380,8 @@ void build(GdrvSystem &S)
          S.P[base + ra + (size_t)cb * lda] = v;
          if (a == b) S.P[base + cb + (size_t)ra * lda] = v;
       }
-   S.Pmax.assign((size_t)na * na, 0.0);
-   for (int a = 0; a < na; a++)
-      for (int b = a; b < na; b++) {
-         const size_t base = S.blk[(size_t)a * na + b];
-         const size_t n = (size_t)S.kinds[S.atoms[a].kind].nfun * S.kinds[S.atoms[b].kind].nfun;
-         double m = 0;
-         for (size_t t = 0; t < n; t++) m = std::max(m, std::fabs(S.P[base + t]));
-         S.Pmax[(size_t)a * na + b] = S.Pmax[(size_t)b * na + a] = m;
-      }
+   make_pmax(S);
+   if (p.replicate > 1) replicate_x(S, p.replicate, cedge);
    S.R_cut = p.rcut_A * BOHR_PER_ANG;
 }
 

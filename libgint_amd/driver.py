@@ -23,6 +23,7 @@ class Params(C.Structure):
         ("basis", C.c_int), ("np_min", C.c_int), ("np_max", C.c_int), ("seed", C.c_uint64),
         ("rcut_A", C.c_double), ("eps", C.c_double), ("min_oo_A", C.c_double), ("symmetric", C.c_int),
         ("cell_A", C.c_double), ("np_tm", C.c_int), ("max_cells", C.c_int), ("ks_mix", C.c_int),
+        ("zmin", C.c_double), ("zmax", C.c_double), ("shift_atom", C.c_int), ("shift_cells", C.c_int * 3), ("replicate", C.c_int),
     ]
 
 
@@ -100,7 +101,8 @@ def tcg_table(nder=21):
 class System:
     def __init__(self, config=None, **kw):
         p = dict(n_water=1, n_tm=0, box_A=10.0, image_shells=0, basis=0, np_min=1, np_max=4, seed=0,
-                 rcut_A=6.0, eps=1e-11, min_oo_A=2.4, symmetric=1, cell_A=0.0, np_tm=16, max_cells=0, ks_mix=0)
+                 rcut_A=6.0, eps=1e-11, min_oo_A=2.4, symmetric=1, cell_A=0.0, np_tm=16, max_cells=0, ks_mix=0,
+                 zmin=0.1, zmax=5e3, shift_atom=-1, shift_cells=(0, 0, 0), replicate=1)
         if config is not None:
             p.update(CONFIGS[config])
         p.update(kw)
@@ -110,7 +112,9 @@ class System:
             # molecules in a box_A cube, cell edge 100 A so the minimum image is the identity
             p["cell_A"] = 100.0
         self.params = dict(p)
-        self._p = Params(**{k: p[k] for k, _ in Params._fields_})
+        vals = {k: p[k] for k, _ in Params._fields_}
+        vals["shift_cells"] = (C.c_int * 3)(*p["shift_cells"])
+        self._p = Params(**vals)
         self.h = drv().gdrv_system_new(C.byref(self._p))
         info = (C.c_int64 * 8)()
         drv().gdrv_system_info(self.h, info)
