@@ -86,6 +86,23 @@ void libgint_dispatch(void *handle);
 
 /* ---- this library's own (no counterpart in the reference) -------------- */
 
+/* Integral output, the use of reference src/compute_TRA.cpp:8-56 (whose
+   launch the reference's K path leaves out, src/libGint.cpp:992): after
+   libgint_amd_set_output(h, out, n) the handle keeps an n-double output
+   array on the device, zeroed; libgint_amd_add_tra(h, la, lb, lc, ld, dest),
+   called for the current set quartet in place of that class's add_qrt /
+   add_qrtt calls, sends the class's spherical integrals (every contraction
+   combination) to out[dest ...] in the reference's TRA order
+   [ila, isa, ilb, isb, ilc, isc, ild, isd] (nla*nlb*nlc*nld*Ns doubles,
+   written, not added; no symm_fac or hf_fac factor); the other classes of
+   the set quartet keep their add_qrtt calls.  libgint_amd_get_output
+   finishes the work and copies the array to `out`.  set_output(h, NULL, 0)
+   releases it.  A class of a set quartet with no surviving primitive
+   quartet writes nothing (its integrals are zero). */
+void libgint_amd_set_output(void *handle, double *out, uint64_t n);
+void libgint_amd_add_tra(void *handle, int la, int lb, int lc, int ld, uint64_t dest);
+void libgint_amd_get_output(void *handle);
+
 /* release a handle (the reference never frees one) */
 void libgint_amd_destroy_handle(void *handle);
 /* wait for the handle's device work */

@@ -47,6 +47,9 @@ ABI = {
     "libgint_add_set": (None, [_vp]),
     "libgint_dispatch": (None, [_vp]),
     # this library's own
+    "libgint_amd_set_output": (None, [_vp, _dp, C.c_uint64]),
+    "libgint_amd_add_tra": (None, [_vp, _i, _i, _i, _i, C.c_uint64]),
+    "libgint_amd_get_output": (None, [_vp]),
     "libgint_amd_destroy_handle": (None, [_vp]),
     "libgint_amd_synchronize": (None, [_vp]),
     "libgint_amd_enable_stats": (None, [_vp, _i]),

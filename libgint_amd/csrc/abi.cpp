@@ -106,6 +106,9 @@ void libgint_amd_add_set_blocks(void *h, double symm_fac, int ld_ac, int ld_ad, 
 }
 void libgint_dispatch(void *h) { E(h)->dispatch_all(); }
 
+void libgint_amd_set_output(void *h, double *out, uint64_t n) { E(h)->set_output(out, n); }
+void libgint_amd_add_tra(void *h, int la, int lb, int lc, int ld, uint64_t dest) { E(h)->add_tra(la, lb, lc, ld, dest); }
+void libgint_amd_get_output(void *h) { E(h)->get_output(); }
 void libgint_amd_destroy_handle(void *h) { delete static_cast<Engine *>(h); }
 void libgint_amd_synchronize(void *h) { E(h)->synchronize(); }
 void libgint_amd_enable_stats(void *h, int on) { E(h)->enable_stats(on != 0); }

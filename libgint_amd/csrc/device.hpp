@@ -76,6 +76,10 @@ struct KsGroup {
 static_assert(sizeof(KsGroup) == 64, "KsGroup layout");
 
 constexpr uint32_t KS_EXPLICIT = 1u << 8;
+// integral output instead of a K update: the class's spherical integrals go
+// to the output array at base[0] | base[1] << 32, ordered
+// [ila, isa, ilb, isb, ilc, isc, ild, isd] (reference src/compute_TRA.cpp:36-47)
+constexpr uint32_t KS_OUTPUT = 1u << 9;
 GINT_HDI uint32_t ks_count(uint32_t flags) { return flags >> 16; }
 
 // Every add_qrtt call of one set quartet at once, when the caller lays out
@@ -247,7 +251,18 @@ struct KTargets {
    double *K[2];
    int nspin;
    double hf_fac;
+   double *out;   // integral output (KS_OUTPUT groups)
 };
+
+// position in a class's output block of integral (isa,isb,isc,isd) of
+// contraction (ia,ib,ic,id): the reference's TRA order
+GINT_HDI size_t tra_index(int ia, int isa, int ib, int isb, int ic, int isc, int id, int isd, int nlb, int nlc, int nld,
+                          int nsa, int nsb, int nsc, int nsd)
+{
+   (void)nsa;
+   return ((((((size_t)ia * nsa + isa) * nlb + ib) * nsb + isb) * nlc + ic) * nsc + isc) * (size_t)(nld * nsd) +
+          (size_t)id * nsd + isd;
+}
 
 struct Scratch {
    double *ptr;

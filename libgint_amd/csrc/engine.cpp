@@ -145,7 +145,7 @@ Engine::~Engine()
       if (ti.d_srcmaps) (void)hipFree(ti.d_srcmaps);
    }
    void *bufs[] = {d_data_, d_C0_, d_grid_, d_bw_, d_ftab_, d_cell_, d_neighs_, d_work_,
-                   d_fix_, d_fix_ws_, d_scratch_, d_stats_, d_fixcnt_};
+                   d_fix_, d_fix_ws_, d_scratch_, d_stats_, d_fixcnt_, d_out_};
    for (void *b : bufs)
       if (b) (void)hipFree(b);
    for (void *e : ev_pool_) (void)hipEventDestroy((hipEvent_t)e);
@@ -619,6 +619,47 @@ void Engine::add_set_blocks(double symm_fac, const int ld[4], const unsigned off
    have_blocks_ = true;
 }
 
+void Engine::set_output(double *out, uint64_t n)
+{
+   dispatch_all();
+   synchronize();
+   if (d_out_) HIPCHECK(hipFree(d_out_));
+   d_out_ = nullptr;
+   h_out_ = out;
+   out_n_ = out ? n : 0;
+   if (out_n_) {
+      HIPCHECK(hipMalloc((void **)&d_out_, sizeof(double) * out_n_));
+      HIPCHECK(hipMemset(d_out_, 0, sizeof(double) * out_n_));
+   }
+}
+
+void Engine::add_tra(int la, int lb, int lc, int ld, uint64_t dest)
+{
+   if (!d_out_) fatal("add_tra without an output array (libgint_amd_set_output)", 0);
+   if (la < 0 || la > MAXL || lb < 0 || lb > MAXL || lc < 0 || lc > MAXL || ld < 0 || ld > MAXL)
+      fatal("add_tra: l must be 0..3", la);
+   if (cur_type_ < 0) return;   // no primitive survived: the integrals are zero, as the array starts
+   const TypePlan &tp = types_[cur_type_].plan;
+   const int c = class_code(la, lb, lc, ld);
+   int k = -1;
+   for (int x = 0; x < tp.nclass; x++)
+      if (tp.cls_code[x] == c) k = x;
+   if (k < 0) fatal("add_tra: the set quartet has no such class", c);
+   const uint64_t n = (uint64_t)tp.cls_nl[k] * (uint64_t)(nsph(la) * nsph(lb) * nsph(lc) * nsph(ld));
+   if (dest + n > out_n_) fatal("add_tra: the class's integrals reach beyond the output array", (long long)dest);
+   for (auto &e : pend_out_)
+      if (e.first == c) fatal("add_tra: class given twice in one set quartet", c);
+   pend_out_.push_back({c, dest});
+}
+
+void Engine::get_output()
+{
+   if (!d_out_) fatal("get_output without an output array", 0);
+   dispatch_all();
+   synchronize();
+   HIPCHECK(hipMemcpy(h_out_, d_out_, sizeof(double) * out_n_, hipMemcpyDeviceToHost));
+}
+
 void Engine::add_set()
 {
    if (cur_type_ >= 0) {
@@ -636,8 +677,19 @@ void Engine::add_set()
          const size_t first = Q.ksg.size();
          const size_t nx = Q.ksx.size();
          for (int k = 0; k < tp.nclass; k++) Q.ksg.push_back(compress(pend_[tp.cls_code[k]], tp, k, Q.ksx));
+         for (const auto &e : pend_out_) {
+            for (int k = 0; k < tp.nclass; k++) {
+               if (tp.cls_code[k] != e.first) continue;
+               KsGroup &g = Q.ksg[first + k];
+               if (ks_count(g.flags)) fatal("add_tra and add_qrtt calls for one class of a set quartet", e.first);
+               std::memset(&g, 0, sizeof g);
+               g.base[0] = (uint32_t)(e.second & 0xffffffffu);
+               g.base[1] = (uint32_t)(e.second >> 32);
+               g.flags = KS_OUTPUT | (uint32_t)tp.cls_nl[k] << 16;
+            }
+         }
          KsSet s;
-         if (Q.ksx.size() == nx && set_level(&Q.ksg[first], tp, s)) {
+         if (Q.ksx.size() == nx && pend_out_.empty() && set_level(&Q.ksg[first], tp, s)) {
             Q.ksg.resize(first);
             gr.ks = (uint32_t)Q.kss.size();
             gr.ksmode = KS_SETLEVEL;
@@ -653,6 +705,7 @@ void Engine::add_set()
       fatal("add_set_blocks without a set quartet", 0);
    }
    have_blocks_ = false;
+   pend_out_.clear();
    // calls of classes without integrals in this set quartet add nothing
    for (int c : touched_) pend_[c].clear();
    touched_.clear();
@@ -1069,6 +1122,7 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
    KTargets kt;
    kt.nspin = nspin_;
    kt.hf_fac = hf_fac_;
+   kt.out = d_out_;
    for (int s = 0; s < 2; s++) {
       kt.P[s] = d_P_[s];
       kt.K[s] = d_K_[s];

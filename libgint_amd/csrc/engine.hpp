@@ -129,6 +129,10 @@ class Engine {
    // extension: every add_qrt / add_qrtt call of the current set quartet at
    // once, for K blocks laid out per pair of sets (KsSet)
    void add_set_blocks(double symm_fac, const int ld[4], const unsigned off[4], const bool T[4]);
+   // extension: integral output (the reference's TRA layout) in place of K
+   void set_output(double *out, uint64_t n);
+   void add_tra(int la, int lb, int lc, int ld, uint64_t dest);
+   void get_output();
    void dispatch_all();
    void synchronize();
 
@@ -204,6 +208,10 @@ class Engine {
    std::vector<int> touched_;
    KsSet blocks_{};            // add_set_blocks of the open set quartet
    bool have_blocks_ = false;
+   std::vector<std::pair<int, uint64_t>> pend_out_;   // (class code, destination) of the open set quartet
+   double *h_out_ = nullptr;   // caller's output array (set_output)
+   double *d_out_ = nullptr;
+   uint64_t out_n_ = 0;
    size_t pend_bytes_ = 0;
    uint64_t max_index_ = 0;   // highest P / K element the pending calls reach
    uint64_t plan_max_index_ = 0;

@@ -834,7 +834,8 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
    const int nq = (int)(ks.flags >> 16);
    if (nq == 0) return;
    const bool expl = (ks.flags & KS_EXPLICIT) != 0;
-   const bool reduce = y.reduce && !expl && nq == nq_all;
+   const bool out_mode = (ks.flags & KS_OUTPUT) != 0;
+   const bool reduce = y.reduce && !expl && !out_mode && nq == nq_all;
    for (int l0 = 0; l0 < nq; l0 += y.lb) {
       const int nb = min(y.lb, nq - l0);
       __syncthreads();
@@ -1005,6 +1006,18 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
          for (int idx = tid; idx < nb * Ns; idx += NT) {
             const int l = fdiv(idx, Ns);
             Iall[(l0 + l) * Ns + (idx - l * Ns)] = WS[l * lws + cap + (idx - l * Ns)];
+         }
+         continue;
+      }
+      if (out_mode) {
+         // the integrals themselves, in the reference's TRA order
+         double *O = kt.out + ((size_t)ks.base[0] | (size_t)ks.base[1] << 32);
+         for (int idx = tid; idx < nb * Ns; idx += NT) {
+            const int l = fdiv(idx, Ns), tt = idx - l * Ns;
+            const int q = l0 + l;
+            const int ia = q / (nlb * nlc * nld), ib = (q / (nlc * nld)) % nlb, ic = (q / nld) % nlc, id = q % nld;
+            const int isa = tt / nsbcd, isb = (tt / nscd) % nsb, isc = (tt / nsd) % nsc, isd = tt % nsd;
+            O[tra_index(ia, isa, ib, isb, ic, isc, id, isd, nlb, nlc, nld, nsa, nsb, nsc, nsd)] = WS[l * lws + cap + tt];
          }
          continue;
       }

@@ -472,7 +472,8 @@ __global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, 
          constexpr int n3 = (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1);
          // all of the set quartet's contractions in one pass, with affine
          // offsets: the K updates are summed over its contractions first
-         const bool reduce = !(ks.flags & KS_EXPLICIT) && nq == y.nq_all && nqc == nq;
+         const bool out_mode = (ks.flags & KS_OUTPUT) != 0;
+         const bool reduce = !(ks.flags & (KS_EXPLICIT | KS_OUTPUT)) && nq == y.nq_all && nqc == nq;
          for (int lq = lane; lq < nqc; lq += 64) {
             double *b0 = B0 + lq * S::WB, *b1 = B1 + lq * S::WB;
             sph_stage_lds<la, n0>(ACC + lq * S::WA, b0);
@@ -483,6 +484,19 @@ __global__ __launch_bounds__(64) void k_tqw(int cls_k, DevTables t, BatchDev b, 
             const int l = l0 + lq;
             const uint32_t cd = code[l];
             const long long ia = cd & 255, ib = (cd >> 8) & 255, ic = (cd >> 16) & 255, id = cd >> 24;
+            if (out_mode) {
+               // the integrals themselves, in the reference's TRA order
+               double *O = kt.out + ((size_t)ks.base[0] | (size_t)ks.base[1] << 32);
+#pragma unroll 4
+               for (int tt = 0; tt < NS; tt++) {
+                  const int isa = tt / ((2 * lb + 1) * (2 * lc + 1) * (2 * ld + 1));
+                  const int isb = (tt / ((2 * lc + 1) * (2 * ld + 1))) % (2 * lb + 1);
+                  const int isc = (tt / (2 * ld + 1)) % (2 * lc + 1), isd = tt % (2 * ld + 1);
+                  O[tra_index((int)ia, isa, (int)ib, isb, (int)ic, isc, (int)id, isd, y.nlb, y.nlc, y.nld, 2 * la + 1,
+                              2 * lb + 1, 2 * lc + 1, 2 * ld + 1)] = b1[tt];
+               }
+               continue;
+            }
             size_t off[4];
             uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
             if (ks.flags & KS_EXPLICIT) {

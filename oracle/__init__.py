@@ -35,6 +35,9 @@ SIG = {
     "oracle_add_set": (None, [_vp]),
     "oracle_add_set_blocks": (None, [_vp, C.c_double] + [C.c_int] * 4 + [C.c_uint] * 4 + [C.c_bool] * 4),
     "oracle_dispatch": (None, [_vp]),
+    "oracle_set_output": (None, [_vp, _dp, C.c_uint64]),
+    "oracle_add_tra": (None, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]),
+    "oracle_get_output": (None, [_vp]),
     "oracle_norm_psi": (C.c_double, [_dp, _dp, C.c_int, C.c_int]),
     "oracle_stats": (None, [_vp, C.POINTER(C.c_uint64)]),
     "oracle_exp": (C.c_double, [C.c_double]),

@@ -238,6 +238,8 @@ typedef struct {
    unsigned nQ;
    OKs *ks;
    int nks, capks;
+   int out_mode, nl[4]; /* oracle_add_tra: integrals to out[dest ...] */
+   uint64_t dest;
 } OClass;
 
 /* one class of a closed set quartet waiting to be integrated: its rows (it
@@ -249,6 +251,8 @@ typedef struct {
    unsigned nQ;
    OKs *ks;
    int nks;
+   int out_mode, nl[4];
+   uint64_t dest;
 } OPendClass;
 
 typedef struct {
@@ -288,6 +292,8 @@ typedef struct {
    uint64_t n_slots, n_surv, n_prmq, n_qrtt, checksum;
    OPendSet *pend;
    int npend, cappend;
+   double *out; /* oracle_set_output */
+   uint64_t out_n;
 } Oracle;
 
 static void o_flush(Oracle *o);
@@ -622,6 +628,37 @@ void oracle_add_set_blocks(void *h, double symm_fac, int ld_ac, int ld_ad, int l
                         }
             }
 }
+
+/* integral output in the reference's TRA layout (the product's
+   libgint_amd_set_output / add_tra / get_output) */
+void oracle_set_output(void *h, double *out, uint64_t n)
+{
+   Oracle *o = (Oracle *)h;
+   o_flush(o);
+   o->out = out;
+   o->out_n = n;
+   if (out) memset(out, 0, sizeof(double) * (size_t)n);
+}
+
+void oracle_add_tra(void *h, int la, int lb, int lc, int ld, uint64_t dest)
+{
+   Oracle *o = (Oracle *)h;
+   if (!o->have_cur) return;
+   const int want[4] = {la, lb, lc, ld};
+   OClass *C = &o->cls[cls_index(la, lb, lc, ld)];
+   for (int t = 0; t < 4; t++) {
+      const OSet *S = &o->sets[o->cur[t]];
+      C->nl[t] = 0;
+      for (int u = 0; u < S->nsub; u++)
+         if (S->l[u] == want[t]) C->nl[t] = S->nl[u];
+   }
+   C->nQ = (unsigned)(C->nl[0] * C->nl[1] * C->nl[2] * C->nl[3]);
+   C->out_mode = 1;
+   C->dest = dest;
+   C->used = 1;
+}
+
+void oracle_get_output(void *h) { o_flush((Oracle *)h); }
 
 /* ------------------------------------------------------------------ */
 /* integrals                                                           */
@@ -1172,6 +1209,21 @@ static void o_integrate_set(const Oracle *o, const OPendSet *ps, double *K[2], O
       double *t1 = (double *)malloc(sizeof(double) * (size_t)Nc * 4);
       double *t2 = (double *)malloc(sizeof(double) * (size_t)Nc * 4);
       for (unsigned q = 0; q < C->nQ; q++) o_sph(la, lb, lc, ld, ABCD0 + (size_t)q * Nc, SPH + (size_t)q * Ns, t1, t2);
+      if (C->out_mode) {
+         /* reference src/compute_TRA.cpp:36-47: SPHER [ila,ilb,ilc,ild][isa,isb,isc,isd]
+            to OUT [ila,isa,ilb,isb,ilc,isc,ild,isd] */
+         const int nla = C->nl[0], nlb = C->nl[1], nlc = C->nl[2], nld = C->nl[3];
+         const int nsa = 2 * la + 1, nsb = 2 * lb + 1, nsc = 2 * lc + 1, nsd = 2 * ld + 1;
+         for (int q = 0; q < (int)C->nQ; q++) {
+            const int ia = q / (nlb * nlc * nld), ib = (q / (nlc * nld)) % nlb, ic = (q / nld) % nlc, id = q % nld;
+            for (int t = 0; t < Ns; t++) {
+               const int isa = t / (nsb * nsc * nsd), isb = (t / (nsc * nsd)) % nsb, isc = (t / nsd) % nsc, isd = t % nsd;
+               const size_t j = ((((((size_t)ia * nsa + isa) * nlb + ib) * nsb + isb) * nlc + ic) * nsc + isc) *
+                                   (size_t)(nld * nsd) + (size_t)id * nsd + isd;
+               o->out[C->dest + j] = SPH[(size_t)q * Ns + t];
+            }
+         }
+      }
       for (int e = 0; e < C->nks && (unsigned)e < C->nQ; e++)
          for (int s = 0; s < o->nspin; s++)
             o_ks(&C->ks[e], la, lb, lc, ld, SPH + (size_t)e * Ns, o->P[s], K[s], o->hf_fac);
@@ -1248,6 +1300,10 @@ void oracle_add_set(void *h)
       pc->nks = C->nks;
       pc->ks = (OKs *)malloc(sizeof(OKs) * (size_t)(C->nks ? C->nks : 1));
       memcpy(pc->ks, C->ks, sizeof(OKs) * (size_t)C->nks);
+      pc->out_mode = C->out_mode;
+      memcpy(pc->nl, C->nl, sizeof pc->nl);
+      pc->dest = C->dest;
+      C->out_mode = 0;
       C->nrows = 0;
       C->nks = 0;
       C->nQ = 0;

@@ -52,6 +52,12 @@ void oracle_add_set_blocks(void *h, double symm_fac, int ld_ac, int ld_ad, int l
                            bool Tac, bool Tad, bool Tbc, bool Tbd);
 void oracle_dispatch(void *h);
 
+/* integral output in the reference's TRA layout (the product's
+   libgint_amd_set_output / add_tra / get_output); set_output zeroes out */
+void oracle_set_output(void *h, double *out, uint64_t n);
+void oracle_add_tra(void *h, int la, int lb, int lc, int ld, uint64_t dest);
+void oracle_get_output(void *h);
+
 /* normalisation of one contraction, reference src/extern_functions.cpp:95-110 */
 double oracle_norm_psi(const double *K, const double *z, int np, int l);
 
