@@ -461,6 +461,8 @@ typedef struct {
    // (libgint_amd_add_set_blocks); null when the library has none
    void (*add_set_blocks)(void *, double, int, int, int, int, unsigned, unsigned, unsigned, unsigned, bool, bool,
                           bool, bool);
+   // optional: integral output of one class (libgint_amd_add_tra)
+   void (*add_tra)(void *, int, int, int, int, uint64_t);
 } GdrvAbi;
 
 namespace {
@@ -799,6 +801,61 @@ void gdrv_stream_replay(const GdrvSystem *S, const GdrvStream *st, const GdrvAbi
                emit_set_blocks(*S, abi, h, ij & 0xffff, ij >> 16, kl & 0xffff, kl >> 16, (int)(w & 7));
             else
                emit_set(*S, abi, h, ij & 0xffff, ij >> 16, kl & 0xffff, kl >> 16, (int)(w & 7), s);
+            break;
+         }
+      }
+   }
+}
+
+// the sets (i, j, k, l) of every recorded set quartet, 4 ints each
+void gdrv_stream_sets(const GdrvStream *st, int32_t *out)
+{
+   for (size_t s = 0; s < st->set_word.size(); s++) {
+      size_t i = (size_t)st->set_word[s];
+      while ((st->w[i] >> 30) != OP_SET) i += (st->w[i] >> 30) == OP_PRM ? 1 : 3;
+      const uint32_t ij = st->w[i + 1], kl = st->w[i + 2];
+      out[4 * s + 0] = (int32_t)(ij & 0xffff);
+      out[4 * s + 1] = (int32_t)(ij >> 16);
+      out[4 * s + 2] = (int32_t)(kl & 0xffff);
+      out[4 * s + 3] = (int32_t)(kl >> 16);
+   }
+}
+
+// replay sets [set_begin, set_end) with stride, asking for the integrals of
+// every class instead of K updates: class (sa,sb,sc,sd) of each set quartet
+// in turn (shells in basis order) goes to *dest, which advances by its
+// nla nlb nlc nld Ns integrals
+void gdrv_stream_replay_tra(const GdrvSystem *S, const GdrvStream *st, const GdrvAbi *abi, void *h, int64_t set_begin,
+                            int64_t set_end, int64_t stride, uint64_t *dest)
+{
+   if (stride < 1) stride = 1;
+   set_end = std::min<int64_t>(set_end, (int64_t)st->set_word.size());
+   for (int64_t s = set_begin; s < set_end; s += stride) {
+      size_t i = (size_t)st->set_word[s];
+      for (;;) {
+         const uint32_t w = st->w[i];
+         const uint32_t op = w >> 30;
+         if (op == OP_PRM) {
+            abi->add_prm(h, w & 15, (w >> 4) & 15, (w >> 8) & 15, (w >> 12) & 15);
+            i++;
+         } else if (op == OP_SHELL) {
+            const uint32_t ij = st->w[i + 1], kl = st->w[i + 2];
+            abi->add_shell(h, ij & 0xffff, ij >> 16, kl & 0xffff, kl >> 16, w & 255, (w >> 8) & 255);
+            i += 3;
+         } else {
+            const uint32_t ij = st->w[i + 1], kl = st->w[i + 2];
+            const int at[4] = {(int)(ij & 0xffff), (int)(ij >> 16), (int)(kl & 0xffff), (int)(kl >> 16)};
+            const Basis *B[4];
+            for (int t = 0; t < 4; t++) B[t] = &S->kinds[S->atoms[at[t]].kind];
+            for (auto &a : B[0]->sh)
+               for (auto &b : B[1]->sh)
+                  for (auto &c : B[2]->sh)
+                     for (auto &d : B[3]->sh) {
+                        abi->add_tra(h, a.l, b.l, c.l, d.l, *dest);
+                        *dest += (uint64_t)(a.nl * b.nl * c.nl * d.nl) *
+                                 (uint64_t)((2 * a.l + 1) * (2 * b.l + 1) * (2 * c.l + 1) * (2 * d.l + 1));
+                     }
+            abi->add_set(h);
             break;
          }
       }

@@ -29,7 +29,7 @@ class Params(C.Structure):
 
 class AbiTable(C.Structure):
     _fields_ = [(n, _vp) for n in ("add_prm", "add_shell", "add_qrt", "add_qrtt", "add_set", "set_Atom", "set_Atom_L",
-                                   "add_set_blocks")]
+                                   "add_set_blocks", "add_tra")]
 
 
 # Workloads.  Config 0 has one cell; its cell edge (100 A) only has to make
@@ -66,6 +66,9 @@ def drv():
             "gdrv_stream_free": (None, [_vp]),
             "gdrv_stream_replay": (None, [_vp, _vp, C.POINTER(AbiTable), _vp, C.c_int64, C.c_int64, C.c_int64,
                                           C.c_int]),
+            "gdrv_stream_sets": (None, [_vp, C.POINTER(C.c_int32)]),
+            "gdrv_stream_replay_tra": (None, [_vp, _vp, C.POINTER(AbiTable), _vp, C.c_int64, C.c_int64, C.c_int64,
+                                              C.POINTER(C.c_uint64)]),
             "gdrv_symmetrize": (None, [_vp, _dp]),
             "gdrv_dense": (None, [_vp, _dp, _dp]),
         }
@@ -86,7 +89,8 @@ def abi_table(lib, prefix="libgint_"):
     set-level extension is libgint_amd_add_set_blocks / oracle_add_set_blocks"""
     t = AbiTable()
     for name, _ in AbiTable._fields_:
-        full = ("libgint_amd_" + name) if (name == "add_set_blocks" and prefix == "libgint_") else prefix + name
+        own = name in ("add_set_blocks", "add_tra")   # this library's extensions
+        full = ("libgint_amd_" + name) if (own and prefix == "libgint_") else prefix + name
         setattr(t, name, C.cast(getattr(lib, full), _vp).value)
     return t
 
@@ -190,6 +194,22 @@ class System:
         return Z[:np_].copy(), shells
 
 
+def tra_size(system, sets):
+    """output doubles of the integral-output replay of the set quartets `sets`
+    (rows of (i, j, k, l)): every class of each, all contractions"""
+    _, kind, _ = system.atoms()
+    shells = {k: system.kind_basis(k)[1] for k in set(kind.tolist())}
+    n = 0
+    for q in sets:
+        per = [[(l, nl) for (l, nl, _) in shells[int(kind[a])]] for a in q]
+        for (la, na) in per[0]:
+            for (lb, nb) in per[1]:
+                for (lc, nc) in per[2]:
+                    for (ld, nd) in per[3]:
+                        n += na * nb * nc * nd * (2 * la + 1) * (2 * lb + 1) * (2 * lc + 1) * (2 * ld + 1)
+    return n
+
+
 class Stream:
     def __init__(self, system, handle):
         self.system = system
@@ -203,6 +223,21 @@ class Stream:
             drv().gdrv_stream_free(self.h)
         except Exception:
             pass
+
+    def sets(self):
+        """(i, j, k, l) of every set quartet, in stream order"""
+        out = np.zeros((self.n_sets, 4), dtype=np.int32)
+        drv().gdrv_stream_sets(self.h, out.ctypes.data_as(C.POINTER(C.c_int32)))
+        return out
+
+    def replay_tra(self, table, handle, begin=0, end=None, stride=1, dest=0):
+        """the set quartets' integrals of every class into the output array
+        (libgint_amd_add_tra), class after class from `dest`; returns the end"""
+        if end is None:
+            end = self.n_sets
+        d = C.c_uint64(dest)
+        drv().gdrv_stream_replay_tra(self.system.h, self.h, C.byref(table), handle, begin, end, stride, C.byref(d))
+        return d.value
 
     def replay(self, table, handle, begin=0, end=None, stride=1, blocks=False):
         """set quartets [begin, end) with stride; blocks=True sends each set
@@ -275,6 +310,23 @@ def kbuild(lib, prefix, system, stream, begin=0, end=None, stride=1, K0=None, pl
     K = np.zeros(system.P_size)
     getattr(lib, prefix + "get_K")(h, _ptr(K))
     return K, h
+
+
+def integrals(lib, prefix, system, stream, begin=0, end=None, stride=1, **kw):
+    """the spherical integrals of every class of the sampled set quartets,
+    through the integral-output extension, in the reference's TRA layout,
+    class after class; returns (array, handle)"""
+    h, table = setup_handle(lib, prefix, system, **kw)
+    if end is None:
+        end = stream.n_sets
+    n = tra_size(system, stream.sets()[begin:end:stride])
+    out = np.zeros(max(n, 1))
+    own = (lambda x: getattr(lib, "libgint_amd_" + x)) if prefix == "libgint_" else (lambda x: getattr(lib, prefix + x))
+    own("set_output")(h, _ptr(out), n)
+    got = stream.replay_tra(table, h, begin, end, stride)
+    assert got == n, (got, n)
+    own("get_output")(h)
+    return out[:n], h
 
 
 def stats(lib, prefix, h):

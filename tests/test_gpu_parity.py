@@ -154,10 +154,37 @@ def test_plan_replay_new_densities(debug):
         Kp = np.zeros(s.P_size)
         lib.libgint_get_K(h, D._ptr(Kp))
         cur = D.stats(lib, "libgint_", h)
-        step = [cur[1] - prev[1], 0, 0, 0, (cur[4] - prev[4]) % (1 << 64)]
+        step = [0, cur[1] - prev[1], 0, 0, (cur[4] - prev[4]) % (1 << 64)]
         prev = cur
         Ko, ho = D.kbuild(o, "oracle_", s, st, P=P)
         so = D.stats(o, "oracle_", ho)
         o.oracle_destroy_handle(ho)
         _check(Kp, Ko, step, so, "replay%d_debug%d" % (i, debug))
     lib.libgint_amd_destroy_handle(h)
+
+
+TRA_CASES = [
+    ("water2_pbc27_tra", dict(n_water=2, box_A=7.0, image_shells=1, np_max=3, seed=25)),
+    ("water1_tm1_pbc27_tra", dict(n_water=1, n_tm=1, box_A=6.0, image_shells=1, np_max=2, np_tm=2, seed=26)),
+]
+
+
+@pytest.mark.parametrize("name,sysargs", TRA_CASES, ids=[c[0] for c in TRA_CASES])
+def test_integral_output_parity(name, sysargs):
+    """integral output (the reference's TRA layout) of every class of every
+    set quartet: the HIP consumers against the oracle, element by element"""
+    from libgint_amd import driver as D, load
+
+    lib = load()
+    o = oracle()
+    s = D.System(**sysargs)
+    st = s.record()
+    Ip, hp = D.integrals(lib, "libgint_", s, st)
+    Io, ho = D.integrals(o, "oracle_", s, st)
+    lib.libgint_amd_destroy_handle(hp)
+    o.oracle_destroy_handle(ho)
+    scale = np.max(np.abs(Io))
+    err = np.max(np.abs(Ip - Io))
+    print(name, "integrals", Io.size, "max|I|", scale, "max|dI|", err, flush=True)
+    assert Io.size > 0 and scale > 0
+    assert err <= KTOL_ABS + KTOL_REL * scale, (name, err)
