@@ -2,18 +2,25 @@
 
 python bench.py [--config C] [--steps K] [--warmup W] [--gpus 1]
 
-A step is one truncated-Coulomb exchange (K) build: set_P, execution of the
-workload's screened call stream (recorded once through the libgint_* C-ABI
-and kept resident in HBM), get_K (H2D of P and D2H of K included).  The
-metric counts surviving (primitive quartet, lattice cell, class) slots
-(Fm[0] > 1e-12, the reference's per-class recurrence work unit) per second;
-K-build seconds per step is ms_per_step.  The CPU baseline is the FP64
-oracle on a strided sample of the same stream, on this host.
+A step is one truncated-Coulomb exchange (K) build of the workload: set_P,
+execution of its screened call stream (recorded once and kept resident in
+HBM as a plan), get_K (H2D of P and D2H of K included).  The timed steps run
+with no instrumentation; one further step, serialised and instrumented,
+gives the survivor count and the per-kernel times of the roofline.
+
+value = surviving (primitive quartet, lattice cell, class) slots per second,
+the reference's per-class Fm / VRR work unit (Fm[0] > 1e-12); ms_per_step is
+K-build seconds per SCF step.  Beside it: the per-step cost of the call
+stream through the libgint_* C-ABI (what a CP2K caller pays every step,
+estimated from a strided sample of set quartets), max|dK| against the FP64
+oracle on a strided sample, and the oracle's throughput on that sample on
+this host's cores (the CPU baseline).
 """
 import argparse
 import ctypes as C
 import json
 import os
+import platform
 import sys
 import time
 
@@ -22,8 +29,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector and FP64 matrix, dense
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 vector (and FP64 matrix), dense
 PEAK_HBM_GBS = 8000.0
+METRIC = "K-build s/SCF step + quartets/s at max|dK|<=1e-8 Ha; p50/p99 request latency"
+
+
+def log(*a):
+    print("#", *a, file=sys.stderr, flush=True)
 
 
 def measured_traffic(kernel, workload):
@@ -42,14 +54,30 @@ def measured_traffic(kernel, workload):
     return None, None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1)
-    ap.add_argument("--cpu-sample-sets", type=int, default=0, help="sets in the CPU sample (0: auto)")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--stride", type=int, default=1, help="every n-th set quartet of the workload (1: all)")
+    ap.add_argument("--abi-stride", type=int, default=64,
+                    help="per-call ABI ingestion is timed on every n-th set quartet and scaled up")
+    ap.add_argument("--check-stride", type=int, default=0,
+                    help="max|dK| and the CPU baseline use every n-th set quartet (0: sized to ~10 s of CPU)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the oracle (max|dK| and CPU baseline)")
+    ap.add_argument("--no-abi", action="store_true", help="skip the per-call ABI ingestion estimate")
     ap.add_argument("--c0-file", default=None, help="CP2K t_c_g.dat to use in place of the generated table")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -80,37 +108,55 @@ def main():
     s = D.System(a.config)
     st = s.record()
     t_rec = time.time() - t0
-    print(f"# config {a.config}: atoms {s.natom} nbf {s.nbf} cells {s.Ng} sets {st.n_sets} rows {st.n_rows} "
-          f"primitive quartets {st.n_prm} contracted quartets {st.n_qrtt} (screened in {t_rec:.1f}s)",
-          file=sys.stderr, flush=True)
+    n_sets = len(range(0, st.n_sets, a.stride))
+    log(f"config {a.config}: {s.describe()}; sets {st.n_sets} rows {st.n_rows} primitive quartets {st.n_prm} "
+        f"contracted quartets {st.n_qrtt} (screened in {t_rec:.1f}s); stride {a.stride}: {n_sets} set quartets")
     P = s.P()
-    h, table = D.setup_handle(lib, "libgint_", s, P=P, max_mem_mib=a.max_mem, stats=True, timing=True, c0=c0)
-    K = np.zeros(s.P_size)
     dp = D._ptr
-    # the screened call stream goes through the ABI once and stays resident
+
+    # the call stream through the reference ABI (add_qrt / add_qrtt per
+    # contracted quartet), as a CP2K caller sends it every step: timed on a
+    # strided sample, host only (recording, nothing reaches the device)
+    abi = None
+    if not a.no_abi:
+        h0, t0tab = D.setup_handle(lib, "libgint_", s, P=P, max_mem_mib=a.max_mem, c0=c0)
+        lib.libgint_amd_plan_begin(h0)
+        sstride = a.stride * a.abi_stride
+        t = time.perf_counter()
+        st.replay(t0tab, h0, 0, st.n_sets, sstride, False)
+        t_abi = time.perf_counter() - t
+        nsmp = len(range(0, st.n_sets, sstride))
+        lib.libgint_amd_destroy_handle(h0)
+        abi = {"s_per_step_est": round(t_abi * n_sets / max(nsmp, 1), 2), "threads": 1,
+               "sample": f"every {sstride}th set quartet ({nsmp}), per-call add_qrt / add_qrtt, {t_abi:.2f} s, "
+                         f"scaled to {n_sets}"}
+        log(f"ABI ingestion: {abi['sample']} -> {abi['s_per_step_est']} s per step on one thread")
+
+    h, table = D.setup_handle(lib, "libgint_", s, P=P, max_mem_mib=a.max_mem, c0=c0)
+    K = np.zeros(s.P_size)
     t0 = time.time()
     lib.libgint_amd_plan_begin(h)
-    nsl = 16
+    nsl = 8
     for i in range(nsl):
-        st.replay(table, h, st.n_sets * i // nsl, st.n_sets * (i + 1) // nsl)
-        print(f"# ingested {i + 1}/{nsl} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+        st.replay(table, h, st.n_sets * i // nsl, st.n_sets * (i + 1) // nsl, a.stride, True)
+        log(f"ingested {i + 1}/{nsl} ({time.time() - t0:.1f}s)")
     lib.libgint_amd_plan_end(h)
     t_ing = time.time() - t0
     info = (C.c_uint64 * 6)()
     lib.libgint_amd_plan_info(h, info)
-    print(f"# ABI ingestion {t_ing:.1f}s, resident plan {info[0] / 2**30:.2f} GiB, {info[1]} quartet types, "
-          f"{info[2]} chunks", file=sys.stderr, flush=True)
+    log(f"set-level ingestion {t_ing:.1f}s, resident plan {info[0] / 2**30:.2f} GiB, {info[1]} quartet types, "
+        f"{info[2]} chunks")
 
-    def step():
-        lib.libgint_set_P(h, dp(P), s.P_size)
+    def step(PP=P, out=K):
+        lib.libgint_set_P(h, dp(PP), s.P_size)
         lib.libgint_amd_plan_execute(h)
-        lib.libgint_get_K(h, dp(K))
+        lib.libgint_get_K(h, dp(out))
 
     for w in range(a.warmup):
+        t = time.perf_counter()
         step()
-        print(f"# warmup {w} done", file=sys.stderr, flush=True)
+        log(f"warmup {w} {time.perf_counter() - t:.3f}s")
     lib.libgint_amd_synchronize(h)
-    lib.libgint_amd_reset_counters(h)
     times = []
     for k in range(a.steps):
         lib.libgint_amd_synchronize(h)
@@ -118,74 +164,110 @@ def main():
         step()
         lib.libgint_amd_synchronize(h)
         times.append(time.perf_counter() - t)
-        print(f"# step {k} {times[-1]:.3f}s", file=sys.stderr, flush=True)
+        log(f"step {k} {times[-1]:.3f}s")
+    tot = sum(times)
+
+    # one more step, instrumented and serialised: survivors, kernel times
+    lib.libgint_amd_enable_stats(h, 1)
+    lib.libgint_amd_enable_timing(h, 1)
+    lib.libgint_amd_set_debug(h, 2)
+    lib.libgint_amd_reset_counters(h)
+    Ki = np.zeros(s.P_size)
+    t = time.perf_counter()
+    step(out=Ki)
+    t_ser = time.perf_counter() - t
     stats = D.stats(lib, "libgint_", h)
     tm = (C.c_double * 12)()
     lib.libgint_amd_timing(h, tm)
-    tot = sum(times)
+    log(f"serialised instrumented step {t_ser:.3f}s; kernel ms fm_r {tm[0]:.0f} tq {tm[1]:.0f} gqs {tm[2]:.0f} "
+        f"fixup {tm[3]:.0f}; K equal to the timed steps' to {np.max(np.abs(Ki - K)):.2e}")
     for cat, nm in ((0, "fm_r by type L"), (1, "tq by type L"), (2, "gqs by type L")):
         det = (C.c_double * 256)()
         lib.libgint_amd_timing_detail(h, cat, det)
-        top = sorted(((det[i] / a.steps, i) for i in range(256) if det[i] > 0), reverse=True)[:12]
-        lab = lambda i: f"L{i}"
-        print(f"# {nm}: " + ", ".join(f"{lab(i)} {ms:.1f}ms" for ms, i in top), file=sys.stderr, flush=True)
-    slots = stats[1] / a.steps          # surviving (slot, class) pairs per K-build
+        top = sorted(((det[i], i) for i in range(256) if det[i] > 0), reverse=True)[:12]
+        log(f"{nm}: " + ", ".join(f"L{i} {ms:.0f}ms" for ms, i in top))
+    slots = stats[1]
     value = slots / (tot / a.steps)
     names = ["fm_r", "tq", "gqs", "fixup"]
-    cat = [0, 1, 2, 3]
-    kms = [tm[i] / a.steps for i in cat]
-    kfl = [stats[6] / a.steps, tm[9] / a.steps, tm[10] / a.steps, 0.0]
-    nl = [tm[4 + i] / a.steps for i in cat]
+    kms = [tm[i] for i in range(4)]
+    kfl = [float(stats[6]), tm[9], tm[10], 0.0]
+    nl = [tm[4 + i] for i in range(4)]
     dom = int(np.argmax(kms))
-    # average duration of one launch of the dominant kernel, and its FLOPs per launch
-    per_launch_ms = kms[dom] / max(nl[dom], 1)
     achieved = kfl[dom] / (kms[dom] * 1e-3) / 1e12 if kms[dom] > 0 else 0.0
     traffic, traffic_src = measured_traffic(names[dom], s.describe())
-    roof = {"kernel": names[dom], "bound": "mfma", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
+    roof = {"kernel": names[dom], "bound": "fp64-valu", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": traffic,
-            "traffic_source": traffic_src,
-            "launches_per_step": nl[dom], "avg_launch_ms": round(per_launch_ms, 4),
-            "kernel_ms": {n: round(m, 3) for n, m in zip(names, kms)},
-            "kernel_tflops": {n: round(f / (m * 1e-3) / 1e12, 4) if m > 0 else 0 for n, f, m in zip(names, kfl, kms)}}
+            "traffic_source": traffic_src, "launches_per_step": nl[dom],
+            "avg_launch_ms": round(kms[dom] / max(nl[dom], 1), 4),
+            "timing": "HIP events around each launch on its stream, all launches of the step serialised on one "
+                      "stream (an extra, instrumented step)",
+            "kernel_ms_serialised": {n: round(m, 2) for n, m in zip(names, kms)},
+            "kernel_tflops": {n: round(f / (m * 1e-3) / 1e12, 4) if m > 0 else 0 for n, f, m in zip(names, kfl, kms)},
+            "flops": "fm_r: the reference chain's Fm + VRR FLOPs of the kept (slot, class) pairs; tq, gqs: "
+                     "contraction 2 n_prm ncp + HRR + SPH + 9 Ns per contracted quartet"}
 
-    cpu = None
+    # max|dK| and the CPU baseline: a strided sample of the same stream,
+    # through the product (stream mode) and the FP64 oracle on all cores
+    check = cpu = None
     if not a.no_cpu:
         import oracle
 
         orc = oracle.load()
-        # a strided sample of the same stream, sized to 10-30 s of one core:
-        # a short probe first, then the sample the probe's rate allows
-        def run_sample(stride):
+        cores = len(os.sched_getaffinity(0))
+        omp = int(os.environ.get("OMP_NUM_THREADS", cores))
+
+        def run_oracle(stride):
             ho, to = D.setup_handle(orc, "oracle_", s, P=P, c0=c0)
             t = time.perf_counter()
-            st.replay(to, ho, 0, st.n_sets, stride)
+            st.replay(to, ho, 0, st.n_sets, stride, True)
+            Ko = np.zeros(s.P_size)
+            orc.oracle_get_K(ho, dp(Ko))
             tc = time.perf_counter() - t
             so = D.stats(orc, "oracle_", ho)
             orc.oracle_destroy_handle(ho)
-            return so, tc
+            return Ko, so, tc
 
-        stride = max(1, st.n_sets // (a.cpu_sample_sets or 200))
-        so, tc = run_sample(stride)
-        if not a.cpu_sample_sets and tc < 5.0:
-            sets_per_s = (st.n_sets // stride) / max(tc, 1e-3)
-            # (the probe's set quartets run about twice as fast as the average)
-            stride = max(1, int(st.n_sets / max(1.0, 30.0 * sets_per_s)))
-            so, tc = run_sample(stride)
-        cpu = {"value": round(so[1] / tc, 1), "unit": "quartets/s", "cores": 1, "kind": "port",
-               "sample": f"every {stride}th set quartet of the stream ({st.n_sets // stride} set quartets, "
-                         f"{so[1]} surviving slots, {tc:.1f} s on one core)"}
-        print(f"# cpu baseline: {cpu['sample']}", file=sys.stderr, flush=True)
+        cs = a.check_stride
+        if not cs:
+            cs = max(1, st.n_sets // 400)
+            _, so, tc = run_oracle(cs)
+            if tc < 3.0:
+                cs = max(1, int(cs * max(tc, 0.05) / 10.0))
+        runs = [run_oracle(cs) for _ in range(3)]
+        Ko, so, _ = runs[0]
+        tcs = sorted(r[2] for r in runs)
+        tc = tcs[1]
+        hs, ts = D.setup_handle(lib, "libgint_", s, P=P, max_mem_mib=a.max_mem, c0=c0, stats=True)
+        st.replay(ts, hs, 0, st.n_sets, cs, True)
+        Kp = np.zeros(s.P_size)
+        lib.libgint_get_K(hs, dp(Kp))
+        sp = D.stats(lib, "libgint_", hs)
+        lib.libgint_amd_destroy_handle(hs)
+        err = float(np.max(np.abs(Kp - Ko)))
+        nsmp = len(range(0, st.n_sets, cs))
+        check = {"max_abs_dK": err, "max_abs_K": float(np.max(np.abs(Ko))), "bar_Ha": 1e-8,
+                 "survivors_equal": bool(sp[1] == so[1] and sp[4] == so[4]),
+                 "sample": f"every {cs}th set quartet ({nsmp}), FP64 GPU path vs the FP64 oracle"}
+        cpu = {"value": round(so[1] / tc, 1), "unit": "quartets/s", "cores": omp, "kind": "port",
+               "sample": f"every {cs}th set quartet of the workload ({nsmp} set quartets, {so[1]} surviving slots); "
+                         f"median of 3 runs {tc:.2f} s (runs {', '.join('%.2f' % x for x in tcs)}); "
+                         f"{omp} OpenMP threads of {cores} cores, {cpu_model()}"}
+        log(f"max|dK| {err:.3e} on {check['sample']}; cpu baseline {cpu['sample']}")
 
     out = {
-        "metric": "K-build s/SCF step + quartets/s at max|dK|<=1e-8 Ha; p50/p99 request latency",
-        "value": round(value, 1), "unit": "quartets/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-        "ms_per_step": round(1e3 * tot / a.steps, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": s.describe(), "atoms": s.natom, "nbf": s.nbf, "cells": s.Ng,
-                   "set_quartets": st.n_sets, "primitive_quartets": st.n_prm, "contracted_quartets": st.n_qrtt,
-                   "surviving_slots": int(slots), "abi_ingestion_s": round(t_ing, 2),
+        "metric": METRIC, "value": round(value, 1), "unit": "quartets/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * tot / a.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": s.describe(), "config_index": a.config, "atoms": s.natom, "nbf": s.nbf,
+                   "cells": s.Ng, "set_quartets": n_sets, "stride": a.stride,
+                   "primitive_quartets": st.n_prm if a.stride == 1 else None,
+                   "contracted_quartets": st.n_qrtt if a.stride == 1 else None,
+                   "surviving_slots_per_step": int(slots),
+                   "kbuild_s_per_step": round(tot / a.steps, 4),
+                   "abi_ingestion": abi, "set_level_ingestion_s": round(t_ing, 2),
+                   "resident_plan_GiB": round(info[0] / 2**30, 3),
                    "path": "FP64, resident plan, fused Hermite cell sum per quartet type"},
-        "roofline": roof, "cpu_baseline": cpu,
+        "max_dK": check, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -113,7 +113,8 @@ void libgint_amd_enable_stats(void *handle, int on);
    Fm[0] > 1e-12 (per class, the reference's recurrence work unit),
    [2] primitive quartets x classes, [3] add_qrtt calls, [4] checksum of the
    surviving slots, [5] rows (add_shell calls with primitives), [6] algorithmic
-   FLOPs of the fm_r kernel, [7] 0 */
+   FLOPs of the fm_r kernel: the reference chain's Fm + VRR FLOPs of the
+   (slot, class) pairs kept, [7] 0 */
 void libgint_amd_stats(void *handle, uint64_t out[8]);
 /* per-kernel device time with HIP events on the handle's stream */
 void libgint_amd_enable_timing(void *handle, int on);

@@ -267,9 +267,28 @@ GINT_HDI size_t tra_index(int ia, int isa, int ib, int isb, int ic, int isc, int
 struct Scratch {
    double *ptr;
    size_t doubles;
+   unsigned grid;   // workgroups of a k_gqs part that uses it (each its own slice)
 };
 
 constexpr unsigned SCRATCH_GRID = 2048;
+
+// The no-allocation executor (libgint_amd_plan_execute_noalloc) takes all
+// its device scratch from one static array of NOALLOC_BYTES per GPU:
+//   [0, 256)                      chunk counters (fix-up entries, overflow)
+//   fix-up list                   NOALLOC_FIX entries of 2 words
+//   fix-up workspace              NOALLOC_FIX x (MAX_NCP + 16) doubles
+//   gqs workspace                 NOALLOC_GWS_BYTES
+//   primitive blocks (ac0)        the rest, NOALLOC_AC0_BYTES
+constexpr size_t NOALLOC_BYTES = 176ull << 20;
+constexpr int MAX_NCP = 84 * 84;              // (e0|f0) doubles of an (f f|f f) type
+constexpr size_t NOALLOC_FIX = 256;
+constexpr size_t NOALLOC_OFF_FIX = 256;
+constexpr size_t NOALLOC_OFF_FIXWS = NOALLOC_OFF_FIX + ((2 * 4 * NOALLOC_FIX + 255) & ~(size_t)255);
+constexpr size_t NOALLOC_OFF_GWS = NOALLOC_OFF_FIXWS + ((NOALLOC_FIX * (MAX_NCP + 16) * 8 + 255) & ~(size_t)255);
+constexpr size_t NOALLOC_GWS_BYTES = 32ull << 20;
+constexpr size_t NOALLOC_OFF_AC0 = NOALLOC_OFF_GWS + NOALLOC_GWS_BYTES;
+constexpr size_t NOALLOC_AC0_BYTES = NOALLOC_BYTES - NOALLOC_OFF_AC0;
+static_assert(NOALLOC_AC0_BYTES >= (120ull << 20), "no-allocation scratch layout");
 constexpr unsigned STATS_SLOTS = 4096;   // counter triples the kernels spread their statistics over
 
 // LDS plan of one class in the quartet kernel
@@ -286,6 +305,10 @@ struct CqLayout {
    int part;        // which of the GQS_PARTS concurrent launches takes the class
    int bytes;
 };
+
+// address of the no-allocation scratch array on the current device, looked
+// up (hipGetSymbolAddress) once per device; null if the lookup failed
+void *noalloc_scratch();
 
 // launchers (kernels.hip), asynchronous on `stream`; 0 or a negative code
 // zeroes n words (a kernel, not a memset node, so that the launch sequence

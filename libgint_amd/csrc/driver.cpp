@@ -862,6 +862,72 @@ void gdrv_stream_replay_tra(const GdrvSystem *S, const GdrvStream *st, const Gdr
    }
 }
 
+// the rest of the ABI a CP2K-like OpenMP team calls (reference
+// src/interface_libgint.F90, src/extern_functions.cpp)
+typedef struct {
+   void *(*create_handle)(void);
+   void (*init)(void *);
+   void (*set_Potential_Truncated)(void *, double, double *, int, int);
+   void (*set_hf_fac)(void *, double);
+   void (*set_max_mem)(void *, int);
+   void (*set_cell)(void *, bool, double *, double *);
+   void (*set_neighs)(void *, double *, int);
+   void (*set_P)(void *, double *, int);
+   void (*get_K)(void *, double *);
+   void (*destroy_handle)(void *);
+   void (*plan_begin)(void *);   // optional (null: the stream is executed directly)
+   void (*plan_end)(void *);
+   void (*plan_execute)(void *);
+} GdrvTeamAbi;
+
+// One K-build by an OpenMP team of nthreads, as CP2K drives the library: a
+// handle per thread (reference src/extern_functions.cpp:8), libgint_init,
+// set_P and get_K called by every thread of the team (they are collective:
+// the device P and K are shared, src/libGint.cpp:37-60,441-447,613-635),
+// and thread t sending set quartets t, t + nthreads, ... of the stream.
+// With plan != 0 each thread records its slice as a resident plan and
+// executes it.  K: the team's result (written once); secs[0] the wall time
+// of the team's ingestion (slowest thread), secs[1] of the whole K-build.
+int gdrv_team_kbuild(const GdrvSystem *S, const GdrvStream *st, const GdrvAbi *abi, const GdrvTeamAbi *t,
+                     int nthreads, int blocks, int plan, double *C0, int ld_C0, int c0_size, int max_mem, double *K,
+                     double *secs)
+{
+   if (nthreads < 1) return -1;
+   double ing = 0.0;
+   const double t0 = omp_get_wtime();
+   std::vector<double> P = S->P;
+   double h9[9], hi9[9];
+   std::memcpy(h9, S->cell18, sizeof h9);
+   std::memcpy(hi9, S->cell18 + 9, sizeof hi9);
+   std::vector<double> g = S->neighs;
+   const int nset = (int)st->set_word.size();
+#pragma omp parallel num_threads(nthreads) reduction(max : ing)
+   {
+      void *h = t->create_handle();
+      t->init(h);
+      if (C0) t->set_Potential_Truncated(h, S->R_cut, C0, ld_C0, c0_size);
+      t->set_hf_fac(h, 1.0);
+      t->set_max_mem(h, max_mem);
+      t->set_cell(h, S->p.image_shells > 0, h9, hi9);
+      t->set_neighs(h, g.data(), (int)g.size() / 3);
+      gdrv_emit_basis(S, abi, h);
+      t->set_P(h, P.data(), (int)P.size());
+      const int me = omp_get_thread_num(), nt = omp_get_num_threads();
+      const double ti = omp_get_wtime();
+      if (plan && t->plan_begin) t->plan_begin(h);
+      gdrv_stream_replay(S, st, abi, h, me, nset, nt, blocks);
+      if (plan && t->plan_begin) t->plan_end(h);
+      ing = omp_get_wtime() - ti;
+      if (plan && t->plan_begin) t->plan_execute(h);
+      t->get_K(h, K);
+#pragma omp barrier
+      t->destroy_handle(h);
+   }
+   secs[0] = ing;
+   secs[1] = omp_get_wtime() - t0;
+   return 0;
+}
+
 // symmetric part of K accumulated by the library, in the block layout:
 // diagonal blocks (B + B^T)/2, off-diagonal blocks hold K_ab + K_ba^T -> /2
 void gdrv_symmetrize(const GdrvSystem *S, double *K)

@@ -167,6 +167,8 @@ void Engine::init()
    if (!g_streams[my_thr_]) HIPCHECK(hipStreamCreateWithFlags(&g_streams[my_thr_], hipStreamNonBlocking));
    stream_ = g_streams[my_thr_];
    ensure_side();
+   // the device address of the no-allocation scratch (once per device)
+   noalloc_scratch_ = noalloc_scratch();
 }
 
 void Engine::ensure_side()
@@ -755,6 +757,10 @@ void Engine::plan_end()
    for (auto &d : plan_dev_) d.release();
    plan_dev_.clear();
    upload(q_, plan_dev_, true);
+   // tables and basis data resident now, so that an execution needs no
+   // allocation (plan_execute_noalloc)
+   ensure_tables();
+   upload_data();
    pend_bytes_ = 0;
    plan_max_index_ = max_index_;
    max_index_ = 0;
@@ -917,6 +923,7 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
       TypeDev &D = devs[ty];
       D.type = (int)ty;
       D.chunks.clear();
+      D.chunks_na.clear();
       D.n_groups = D.n_rows = D.n_prm = D.n_ksx = 0;
       if (Q.empty()) continue;
       TypeInfo &ti = types_[ty];
@@ -963,44 +970,57 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
          for (int k = 0; k < tp.nclass; k++)
             if (ks_count(gks(gi, k).flags)) D.class_used[k] = 1;
       }
-      // chunks of whole groups whose primitive blocks fit the work buffer; a
-      // set quartet larger than the buffer gets a chunk of its own, for which
-      // the buffer grows (ensure_work)
-      Chunk c{};
-      size_t bytes = 0;
+      // chunks of whole groups whose primitive blocks fit `wbytes`; a set
+      // quartet larger than the allocating path's buffer gets a chunk of its
+      // own, for which that buffer grows (ensure_work)
+      std::vector<size_t> gbytes(D.n_groups);
       for (size_t gi = 0; gi < D.n_groups; gi++) {
          const GroupRec &g = Q.groups[gi];
          size_t gp = 0;
          for (uint32_t r = g.row0; r < g.row1; r++) gp += Q.rows[r].n_prm;
-         const size_t gb = gp * sizeof(double) * (size_t)tp.ncp;
-         D.max_group_bytes = std::max(D.max_group_bytes, gb);
-         if (bytes + gb > work && c.g1 > c.g0) {
-            D.chunks.push_back(c);
-            c = Chunk{};
-            c.g0 = c.g1 = (uint32_t)gi;
-            c.r0 = c.r1 = g.row0;
-            c.p0 = c.p1 = Q.rows[g.row0].prm_off;
-            bytes = 0;
-         }
-         if (c.g1 == c.g0) {
-            c.g0 = (uint32_t)gi;
-            c.r0 = g.row0;
-            c.p0 = Q.rows[g.row0].prm_off;
-         }
-         c.g1 = (uint32_t)gi + 1;
-         c.r1 = g.row1;
-         c.p1 = (uint32_t)(Q.rows[g.row0].prm_off + gp);
-         bytes += gb;
-         // algorithmic FLOPs of the quartet kernel: contraction, HRR, SPH, KS
-         for (int k = 0; k < tp.nclass; k++) {
-            const double nq = ks_count(gks(gi, k).flags);
-            const ClassPlan &cp = ti.cplans[k];
-            const double f = nq * (2.0 * (double)gp * cp.ncp + (double)(g.row1 - g.row0) * cp.hrr_flops + cp.sph_flops +
-                                   9.0 * cp.Ns);
-            (ti.lays[k].skip ? c.tq_flops : c.k3_flops) += f;
-         }
+         gbytes[gi] = gp * sizeof(double) * (size_t)tp.ncp;
+         D.max_group_bytes = std::max(D.max_group_bytes, gbytes[gi]);
       }
-      if (c.g1 > c.g0) D.chunks.push_back(c);
+      auto make_chunks = [&](size_t wbytes, std::vector<Chunk> &out, bool flops) {
+         Chunk c{};
+         size_t bytes = 0;
+         for (size_t gi = 0; gi < D.n_groups; gi++) {
+            const GroupRec &g = Q.groups[gi];
+            const size_t gb = gbytes[gi];
+            const size_t gp = gb / (sizeof(double) * (size_t)tp.ncp);
+            if (bytes + gb > wbytes && c.g1 > c.g0) {
+               out.push_back(c);
+               c = Chunk{};
+               c.g0 = c.g1 = (uint32_t)gi;
+               c.r0 = c.r1 = g.row0;
+               c.p0 = c.p1 = Q.rows[g.row0].prm_off;
+               bytes = 0;
+            }
+            if (c.g1 == c.g0) {
+               c.g0 = (uint32_t)gi;
+               c.r0 = g.row0;
+               c.p0 = Q.rows[g.row0].prm_off;
+            }
+            c.g1 = (uint32_t)gi + 1;
+            c.r1 = g.row1;
+            c.p1 = (uint32_t)(Q.rows[g.row0].prm_off + gp);
+            bytes += gb;
+            if (!flops) continue;
+            // algorithmic FLOPs of the quartet kernel: contraction, HRR, SPH, KS
+            for (int k = 0; k < tp.nclass; k++) {
+               const double nq = ks_count(gks(gi, k).flags);
+               const ClassPlan &cp = ti.cplans[k];
+               const double f = nq * (2.0 * (double)gp * cp.ncp + (double)(g.row1 - g.row0) * cp.hrr_flops +
+                                      cp.sph_flops + 9.0 * cp.Ns);
+               (ti.lays[k].skip ? c.tq_flops : c.k3_flops) += f;
+            }
+         }
+         if (c.g1 > c.g0) out.push_back(c);
+      };
+      make_chunks(work, D.chunks, true);
+      D.chunks_na.clear();
+      D.na_fit = D.max_group_bytes <= NOALLOC_AC0_BYTES;
+      if (D.na_fit) make_chunks(NOALLOC_AC0_BYTES, D.chunks_na, false);
       n_rows_ += D.n_rows;
       if (release_host) Q.release();
    }
@@ -1076,10 +1096,8 @@ void Engine::harvest_events()
    ev_used_ = 0;
 }
 
-void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
+DevTables Engine::dev_tables() const
 {
-   const TypePlan &tp = ti.plan;
-   hipStream_t st = (hipStream_t)stream_;
    DevTables t;
    t.C0 = d_C0_;
    t.ld_C0 = ld_C0_;
@@ -1092,7 +1110,14 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
    t.Ng = (int)neighs_.size() / 3;
    t.truncated = truncated_ ? 1 : 0;
    t.R_cut = R_cut_;
+   return t;
+}
 
+void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c, const RunBufs &w, const KTargets &kt)
+{
+   const TypePlan &tp = ti.plan;
+   hipStream_t st = (hipStream_t)stream_;
+   const DevTables t = dev_tables();
    BatchDev b;
    b.tp = ti.d_plan;
    b.e1 = ti.d_e1;
@@ -1105,39 +1130,32 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
    b.kss = td.kss;
    b.ksx = td.ksx;
    b.data = d_data_;
-   b.ac0 = d_work_;
+   b.ac0 = w.ac0;
    b.p0 = c.p0;
    b.p1 = c.p1;
    b.r0 = c.r0;
    b.r1 = c.r1;
    b.g0 = c.g0;
    b.g1 = c.g1;
-   b.stats = stats_on_ ? d_stats_ : nullptr;
-   b.fix = d_fix_;
-   b.fix_count = d_fixcnt_;
-   b.fix_cap = (uint32_t)fix_cap_;
+   b.stats = w.stats;
+   b.fix = w.fix;
+   b.fix_count = w.fixcnt;
+   b.fix_cap = w.fix_cap;
    b.force_fix = debug_ & 1;
-   b.fix_ws = d_fix_ws_;
+   b.fix_ws = w.fix_ws;
+   auto evb = [&](int cat, void *q) { if (w.count) ev_begin(cat, q); };
+   auto eve = [&](int cat, void *q) { if (w.count) ev_end(cat, q); };
 
-   KTargets kt;
-   kt.nspin = nspin_;
-   kt.hf_fac = hf_fac_;
-   kt.out = d_out_;
-   for (int s = 0; s < 2; s++) {
-      kt.P[s] = d_P_[s];
-      kt.K[s] = d_K_[s];
-   }
-
-   if (launch_zero_u32(d_fixcnt_, 1, st)) fatal("counter reset launch", 0);
-   ev_begin(0, st);
+   if (launch_zero_u32(w.fixcnt, 1, st)) fatal("counter reset launch", 0);
+   evb(0, st);
    ev_key_ = tp.L;
    int rc = launch_fm_r(tp, t, b, st);
    if (rc) fatal("fm_r launch", rc);
-   ev_end(0, st);
-   ev_begin(3, st);
-   rc = launch_fixup(tp, t, b, (debug_ & 1) ? (uint32_t)fix_cap_ : 64u, st);
+   eve(0, st);
+   evb(3, st);
+   rc = launch_fixup(tp, t, b, (debug_ & 1) ? w.fix_cap : std::min<uint32_t>(64u, w.fix_cap), st);
    if (rc) fatal("fix-up launch", rc);
-   ev_end(3, st);
+   eve(3, st);
    // debug bit 1: every consumer on the main stream, one after the other
    // (kernel durations without overlap, for profiling)
    const bool serial = (debug_ & 2) != 0;
@@ -1150,27 +1168,27 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c)
          any = any || (td.class_used[k] && !ti.lays[k].skip && ti.lays[k].part == part);
       if (!any) continue;
       void *q = (part == 0 || serial) ? (void *)st : side_[(part - 1) % NSIDE];
-      ev_begin(2, q);
+      evb(2, q);
       ev_key_ = tp.L;
-      rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, part, t, b, kt,
-                      Scratch{d_scratch_, scratch_cap_}, q);
+      rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, part, t, b, kt, w.gws, q);
       if (rc) fatal("gqs launch", rc);
-      ev_end(2, q);
+      eve(2, q);
    }
    int si = 0;
    for (int k = 0; k < tp.nclass; k++) {
       if (!td.class_used[k] || !ti.lays[k].skip) continue;
       void *q = serial ? (void *)st : side_[si++ % NSIDE];
-      ev_begin(1, q);
+      evb(1, q);
       ev_key_ = tp.L;
       rc = launch_tq(k, tp, t, b, kt, ti.d_srcmaps + ti.lays[k].o_src, q);
       if (rc) fatal("tq launch", rc);
-      ev_end(1, q);
+      eve(1, q);
    }
    for (int i = 0; i < NSIDE; i++) {
       HIPCHECK(hipEventRecord((hipEvent_t)ev_join_[1 + i], (hipStream_t)side_[i]));
       HIPCHECK(hipStreamWaitEvent(st, (hipEvent_t)ev_join_[1 + i], 0));
    }
+   if (!w.count) return;
    timing_.flops[1] += c.tq_flops;
    timing_.flops[2] += c.k3_flops;
    n_prmc_ += (uint64_t)(c.p1 - c.p0) * tp.nclass;
@@ -1202,8 +1220,90 @@ void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
       HIPCHECK(hipMalloc((void **)&d_scratch_, scr * sizeof(double)));
       scratch_cap_ = scr;
    }
+   RunBufs w;
+   w.ac0 = d_work_;
+   w.fix = d_fix_;
+   w.fixcnt = d_fixcnt_;
+   w.fix_ws = d_fix_ws_;
+   w.fix_cap = (uint32_t)fix_cap_;
+   w.stats = stats_on_ ? d_stats_ : nullptr;
+   w.gws = Scratch{d_scratch_, scratch_cap_, SCRATCH_GRID};
+   w.count = true;
+   KTargets kt;
+   kt.nspin = nspin_;
+   kt.hf_fac = hf_fac_;
+   kt.out = d_out_;
+   for (int s = 0; s < 2; s++) {
+      kt.P[s] = d_P_[s];
+      kt.K[s] = d_K_[s];
+   }
    for (auto &D : devs)
-      for (auto &c : D.chunks) run_chunk(types_[D.type], D, c);
+      for (auto &c : D.chunks) run_chunk(types_[D.type], D, c, w, kt);
+}
+
+// The resident plan, like plan_execute, but into the caller's device P and
+// K (K is added to, not zeroed) and with every scratch buffer carved from
+// the static no-allocation array: no device allocation, no memset node.
+int Engine::plan_execute_noalloc(const double *Pa, const double *Pb, double *Ka, double *Kb, uint64_t n)
+{
+   auto err = [](int code, const char *what, long long v) {
+      std::fprintf(stderr, "libgint: plan_execute_noalloc: %s (%lld)\n", what, v);
+      std::fflush(stderr);
+      return code;
+   };
+   if (recording_) return err(-1, "called while recording a plan", 0);
+   if (!stream_) return err(-1, "libgint_init has not been called", 0);
+   if (!Pa || !Ka || (!Pb) != (!Kb)) return err(-2, "P and K device arrays are required (both spins or neither)", 0);
+   if (cur_type_ >= 0 || !touched_.empty()) return err(-1, "a set quartet is open: call add_set first", 0);
+   bool any = false;
+   size_t ncp_max = 0;
+   for (auto &D : plan_dev_) {
+      if (D.chunks_na.empty()) continue;
+      any = true;
+      const TypeInfo &ti = types_[D.type];
+      if (!D.na_fit)
+         return err(-4, "a set quartet's primitive blocks exceed the no-allocation scratch (bytes of the largest)",
+                    (long long)D.max_group_bytes);
+      if (truncated_ && ti.plan.L > nder_) return err(-7, "the C0 table has too few derivative orders", ti.plan.L);
+      if (ti.cq_gws && NOALLOC_GWS_BYTES / 8 / ((size_t)GQS_PARTS * ti.cq_gws) < 1)
+         return err(-5, "a class workspace exceeds the no-allocation scratch", ti.cq_gws);
+      ncp_max = std::max(ncp_max, (size_t)ti.plan.ncp);
+   }
+   if (!any) return 0;
+   if (plan_max_index_ >= n) return err(-3, "the plan's add_qrtt offsets reach beyond n", (long long)plan_max_index_);
+   if (tables_dirty_ || !d_grid_ || data_version_dev_ != store_.version())
+      return err(-6, "tables or basis data changed since libgint_amd_plan_end: record the plan again", 0);
+   if (ncp_max > (size_t)MAX_NCP) return err(-4, "block size beyond the fix-up workspace", (long long)ncp_max);
+   char *base = (char *)noalloc_scratch_;
+   if (!base) return err(-8, "the no-allocation scratch array has no device address", 0);
+   RunBufs w;
+   w.ac0 = (double *)(base + NOALLOC_OFF_AC0);
+   w.fix = (uint32_t *)(base + NOALLOC_OFF_FIX);
+   w.fixcnt = (uint32_t *)base;
+   w.fix_ws = (double *)(base + NOALLOC_OFF_FIXWS);
+   w.fix_cap = (uint32_t)NOALLOC_FIX;
+   w.stats = nullptr;
+   w.count = false;
+   KTargets kt;
+   kt.nspin = Pb ? 2 : 1;
+   kt.hf_fac = hf_fac_;
+   kt.out = d_out_;
+   kt.P[0] = Pa;
+   kt.P[1] = Pb;
+   kt.K[0] = Ka;
+   kt.K[1] = Kb;
+   // the overflow word (it is only read back by the allocating path)
+   if (launch_zero_u32(w.fixcnt, 2, stream_)) fatal("counter reset launch", 0);
+   for (auto &D : plan_dev_) {
+      if (D.chunks_na.empty()) continue;
+      const TypeInfo &ti = types_[D.type];
+      const unsigned g = ti.cq_gws ? (unsigned)std::min<size_t>(SCRATCH_GRID, NOALLOC_GWS_BYTES / 8 /
+                                                                                ((size_t)GQS_PARTS * ti.cq_gws))
+                                   : SCRATCH_GRID;
+      w.gws = Scratch{(double *)(base + NOALLOC_OFF_GWS), NOALLOC_GWS_BYTES / 8, g};
+      for (auto &c : D.chunks_na) run_chunk(ti, D, c, w, kt);
+   }
+   return 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -76,6 +76,8 @@ struct TypeDev {
    size_t n_rows = 0, n_groups = 0, n_prm = 0, n_ksx = 0, n_ksg = 0, n_kss = 0;
    size_t max_group_bytes = 0;            // largest set quartet's primitive blocks
    std::vector<Chunk> chunks;
+   std::vector<Chunk> chunks_na;          // the same work in chunks that fit the no-allocation scratch
+   bool na_fit = true;                    // every set quartet fits it
    std::vector<char> class_used;
    void release();
 };
@@ -93,6 +95,17 @@ struct TypeInfo {
    ClassPlan *d_cps = nullptr;
    CqLayout *d_lays = nullptr;
    int *d_srcmaps = nullptr;
+};
+
+// device buffers of one chunk's kernels
+struct RunBufs {
+   double *ac0;
+   uint32_t *fix, *fixcnt;
+   double *fix_ws;
+   uint32_t fix_cap;
+   unsigned long long *stats;
+   Scratch gws;
+   bool count;   // host counters and timing events
 };
 
 struct Timing {
@@ -147,6 +160,9 @@ class Engine {
    void plan_begin();
    void plan_end();
    void plan_execute();
+   // extension: the resident plan into caller-owned device P / K, with every
+   // scratch buffer in the library's static no-allocation array; 0 or < 0
+   int plan_execute_noalloc(const double *Pa, const double *Pb, double *Ka, double *Kb, uint64_t n);
    void plan_clear();
    void plan_info(uint64_t out[6]);
    void queue_info(uint64_t out[8]);
@@ -161,7 +177,8 @@ class Engine {
    bool set_level(const KsGroup *g, const TypePlan &tp, KsSet &s) const;
    void upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool release_host);
    void execute(std::vector<TypeDev> &devs, uint64_t max_index);
-   void run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c);
+   void run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c, const RunBufs &w, const KTargets &kt);
+   DevTables dev_tables() const;
    void ensure_work(size_t ncp_max, size_t max_prm_chunk, size_t max_group_bytes);
    void ensure_tables();
    void upload_data();
@@ -234,6 +251,7 @@ class Engine {
    size_t fix_cap_ = 0, fix_ws_cap_ = 0;
    double *d_scratch_ = nullptr;
    size_t scratch_cap_ = 0;
+   void *noalloc_scratch_ = nullptr;   // this device's static scratch array (init)
    unsigned long long *d_stats_ = nullptr;   // [0..2] kernel statistics, [3] unused
    uint32_t *d_fixcnt_ = nullptr;             // [0] entries of the chunk, [1] slots beyond the list (sticky)
    uint64_t fix_overflow_ = 0;                // such slots seen so far (kept by every class)

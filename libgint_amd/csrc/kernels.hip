@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "device.hpp"
 #include "gint_math.h"
@@ -731,7 +732,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                atomicAdd(&st[0], n_surv);
                atomicAdd(&st[1], csum);
             }
-            atomicAdd(&st[2], n_any * (unsigned long long)tp.surv_flops + (unsigned long long)(tp.prim_flops * nprim));
+            // algorithmic FLOPs: the reference chain's Fm and VRR of every
+            // (slot, class) pair kept (reference src/compute_Fm.cpp,
+            // src/compute_VRR.cpp; summed over the type's classes)
+            atomicAdd(&st[2], n_any * (unsigned long long)tp.ref_slot_flops);
          }
       }
    }
@@ -1309,10 +1313,12 @@ int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lay
    unsigned grid = std::min(ng, 1u << 20);
    double *gws = nullptr;
    if (gws_stride) {
-      // each part has its own workspace: the parts run concurrently
-      grid = std::min(grid, SCRATCH_GRID);
-      if (part < 0 || part >= GQS_PARTS || !s.ptr || s.doubles < (size_t)GQS_PARTS * gws_stride * SCRATCH_GRID) return -2;
-      gws = s.ptr + (size_t)part * gws_stride * SCRATCH_GRID;
+      // each part has its own workspace (the parts run concurrently), one
+      // slice per workgroup
+      const unsigned gcap = s.grid ? std::min(s.grid, SCRATCH_GRID) : SCRATCH_GRID;
+      grid = std::min(grid, gcap);
+      if (part < 0 || part >= GQS_PARTS || !s.ptr || s.doubles < (size_t)GQS_PARTS * gws_stride * gcap) return -2;
+      gws = s.ptr + (size_t)part * gws_stride * gcap;
    }
    // a type whose workspace leaves room for few workgroups per CU gets wider
    // ones, so that more wavefronts hide the memory latency
@@ -1323,6 +1329,30 @@ int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lay
       hipLaunchKernelGGL(k_gqs<128>, dim3(grid), dim3(128), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
                          b.tp, t, b, kt, gws, gws_stride, cls_bytes, part);
    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------
+// scratch of the no-allocation executor: one array per GPU, held for the
+// life of the process.  In host code `scratch` only names the symbol; its
+// device address comes from hipGetSymbolAddress, per device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[NOALLOC_BYTES / sizeof(double)];
+
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+void *noalloc_scratch()
+{
+   int dev = 0;
+   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+   std::lock_guard<std::mutex> lock(g_scratch_mu);
+   if (!g_scratch_ptr[dev]) {
+      void *ptr = nullptr;
+      if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+      g_scratch_ptr[dev] = ptr;
+   }
+   return g_scratch_ptr[dev];
 }
 
 __global__ void k_zero_u32(uint32_t *p, int n)

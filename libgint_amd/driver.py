@@ -69,6 +69,8 @@ def drv():
             "gdrv_stream_sets": (None, [_vp, C.POINTER(C.c_int32)]),
             "gdrv_stream_replay_tra": (None, [_vp, _vp, C.POINTER(AbiTable), _vp, C.c_int64, C.c_int64, C.c_int64,
                                               C.POINTER(C.c_uint64)]),
+            "gdrv_team_kbuild": (C.c_int, [_vp, _vp, C.POINTER(AbiTable), C.POINTER(TeamAbi), C.c_int, C.c_int,
+                                           C.c_int, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp]),
             "gdrv_symmetrize": (None, [_vp, _dp]),
             "gdrv_dense": (None, [_vp, _dp, _dp]),
         }
@@ -310,6 +312,37 @@ def kbuild(lib, prefix, system, stream, begin=0, end=None, stride=1, K0=None, pl
     K = np.zeros(system.P_size)
     getattr(lib, prefix + "get_K")(h, _ptr(K))
     return K, h
+
+
+class TeamAbi(C.Structure):
+    _fields_ = [(n, _vp) for n in ("create_handle", "init", "set_Potential_Truncated", "set_hf_fac", "set_max_mem",
+                                   "set_cell", "set_neighs", "set_P", "get_K", "destroy_handle", "plan_begin",
+                                   "plan_end", "plan_execute")]
+
+
+def team_kbuild(lib, system, stream, nthreads, blocks=False, plan=False, truncated=True, max_mem_mib=4096):
+    """one K-build by an OpenMP team of `nthreads` threads, one handle each,
+    collective init / set_P / get_K, thread t sending set quartets
+    t, t + nthreads, ...  (the product library only); returns K and
+    (ingestion seconds of the slowest thread, K-build seconds)"""
+    t = TeamAbi()
+    own = {"destroy_handle", "plan_begin", "plan_end", "plan_execute"}
+    for name, _ in TeamAbi._fields_:
+        full = ("libgint_amd_" if name in own else "libgint_") + name
+        setattr(t, name, C.cast(getattr(lib, full), _vp).value)
+    table = abi_table(lib)
+    K = np.zeros(system.P_size)
+    secs = np.zeros(2)
+    c0 = ld = None
+    if truncated:
+        c0, ld = tcg_table()
+        c0 = np.ascontiguousarray(c0)
+    rc = drv().gdrv_team_kbuild(system.h, stream.h, C.byref(table), C.byref(t), nthreads, 1 if blocks else 0,
+                                1 if plan else 0, _ptr(c0.ravel()) if truncated else None, ld or 0,
+                                c0.size if truncated else 0, max_mem_mib, _ptr(K), _ptr(secs))
+    if rc != 0:
+        raise RuntimeError("gdrv_team_kbuild failed: %d" % rc)
+    return K, (float(secs[0]), float(secs[1]))
 
 
 def integrals(lib, prefix, system, stream, begin=0, end=None, stride=1, **kw):
