@@ -469,9 +469,22 @@ namespace {
 
 constexpr uint32_t OP_PRM = 0u, OP_SHELL = 1u, OP_SET = 2u;
 
+// a set quartet (i,j,k,l) belongs to the sample of 1 in `sample` when this
+// hash of its indices is a multiple of `sample` (independent of the order
+// in which the stream is recorded)
+inline bool in_sample(int i, int j, int k, int l, int64_t sample)
+{
+   if (sample <= 1) return true;
+   uint64_t z = ((uint64_t)(uint32_t)pack16(i, j) << 32 | pack16(k, l)) + 0x9E3779B97F4A7C15ULL;
+   z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+   z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+   z ^= z >> 31;
+   return z % (uint64_t)sample == 0;
+}
+
 void record_bra(const GdrvSystem &S, const std::vector<std::vector<PairImage>> &pairs,
                 const std::vector<std::pair<int, int>> &plist, size_t ib, std::vector<uint32_t> &out,
-                int64_t cnt[4])
+                int64_t cnt[4], int64_t sample)
 {
    const int na = (int)S.atoms.size();
    const double eps = S.p.eps;
@@ -485,7 +498,7 @@ void record_bra(const GdrvSystem &S, const std::vector<std::vector<PairImage>> &
    for (size_t kb = 0; kb < kend; kb++) {
       const int k = plist[kb].first, l = plist[kb].second;
       const std::vector<PairImage> &KI = pairs[kb];
-      if (KI.empty()) continue;
+      if (KI.empty() || !in_sample(i, j, k, l, sample)) continue;
       const double Pm = std::max(std::max(S.Pmax[(size_t)i * na + k], S.Pmax[(size_t)i * na + l]),
                                  std::max(S.Pmax[(size_t)j * na + k], S.Pmax[(size_t)j * na + l]));
       double qket = 0;
@@ -704,7 +717,9 @@ void gdrv_emit_basis(const GdrvSystem *S, const GdrvAbi *abi, void *h)
    }
 }
 
-GdrvStream *gdrv_stream_record(const GdrvSystem *S, int nthreads)
+// the screened call stream of the system; sample > 1 keeps only the set
+// quartets of the hash sample of 1 in `sample` (in_sample)
+GdrvStream *gdrv_stream_record(const GdrvSystem *S, int nthreads, int64_t sample)
 {
    const int na = (int)S->atoms.size();
    std::vector<std::pair<int, int>> plist;
@@ -729,7 +744,7 @@ GdrvStream *gdrv_stream_record(const GdrvSystem *S, int nthreads)
 #pragma omp parallel for schedule(dynamic, 1)
    for (long b = 0; b < (long)plist.size(); b++) {
       int64_t c[4] = {0, 0, 0, 0};
-      record_bra(*S, pairs, plist, (size_t)b, chunks[b], c);
+      record_bra(*S, pairs, plist, (size_t)b, chunks[b], c, sample);
       for (int q = 0; q < 4; q++) counts[b][q] = c[q];
    }
    GdrvStream *st = new GdrvStream();

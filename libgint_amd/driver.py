@@ -61,7 +61,7 @@ def drv():
             "gdrv_system_atoms": (None, [_vp, _dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
             "gdrv_kind_basis": (C.c_int, [_vp, C.c_int, _dp, C.POINTER(C.c_int), _dp]),
             "gdrv_emit_basis": (None, [_vp, C.POINTER(AbiTable), _vp]),
-            "gdrv_stream_record": (_vp, [_vp, C.c_int]),
+            "gdrv_stream_record": (_vp, [_vp, C.c_int, C.c_int64]),
             "gdrv_stream_info": (None, [_vp, _i64p]),
             "gdrv_stream_free": (None, [_vp]),
             "gdrv_stream_replay": (None, [_vp, _vp, C.POINTER(AbiTable), _vp, C.c_int64, C.c_int64, C.c_int64,
@@ -161,8 +161,11 @@ class System:
     def rcut(self):
         return drv().gdrv_system_rcut(self.h)
 
-    def record(self, nthreads=0):
-        return Stream(self, drv().gdrv_stream_record(self.h, nthreads))
+    def record(self, nthreads=0, sample=1):
+        """the screened call stream; sample > 1 keeps the set quartets
+        (i,j,k,l) whose index hash is a multiple of `sample` (about 1 in
+        `sample`, the same ones on every machine)"""
+        return Stream(self, drv().gdrv_stream_record(self.h, nthreads, sample))
 
     def symmetrize(self, K):
         K = np.ascontiguousarray(K, dtype=np.float64).copy()
