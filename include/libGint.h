@@ -139,6 +139,28 @@ void libgint_amd_plan_begin(void *handle);
 void libgint_amd_plan_end(void *handle);
 void libgint_amd_plan_execute(void *handle);
 void libgint_amd_plan_clear(void *handle);
+/* No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): the handle's resident plan, as libgint_amd_plan_execute
+   runs it, but with the density read from the caller's device array P
+   (Pb: spin b, or NULL) and the exchange terms ADDED into the caller's
+   device array K (Kb), n doubles each (K is not zeroed).  Every scratch
+   buffer of the kernels (primitive blocks, fix-up list, class workspace,
+   counters) is carved from one static device array of 176 MiB per GPU,
+   held until the process ends; the call allocates and frees nothing.  It
+   is asynchronous on the handle's stream (libgint_amd_synchronize waits).
+   Calls on one GPU share that array, which keeps the last call's contents:
+   calls that overlap in time on one GPU (other handles, threads or streams)
+   race and give wrong results, so serialise them.  Returns 0, or, with a
+   message on stderr and nothing launched: -1 protocol (recording, no
+   libgint_init, an open set quartet), -2 missing arrays, -3 the plan's
+   offsets reach beyond n, -4 one set quartet's primitive blocks exceed the
+   array's 130 MiB block region (libgint_amd_plan_execute takes those),
+   -5 a class workspace beyond its 32 MiB region, -6 tables or basis data
+   changed since libgint_amd_plan_end, -7 the C0 table has too few orders,
+   -8 the array has no device address. */
+int libgint_amd_plan_execute_noalloc(void *handle, const double *P, const double *Pb, double *K, double *Kb,
+                                     uint64_t n);
 /* out[0] resident bytes, [1] quartet types, [2] chunks, [3] set quartets,
    [4] rows, [5] primitive quartets */
 void libgint_amd_plan_info(void *handle, uint64_t out[6]);

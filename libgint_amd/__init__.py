@@ -62,6 +62,7 @@ ABI = {
     "libgint_amd_plan_begin": (None, [_vp]),
     "libgint_amd_plan_end": (None, [_vp]),
     "libgint_amd_plan_execute": (None, [_vp]),
+    "libgint_amd_plan_execute_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, C.c_uint64]),
     "libgint_amd_plan_clear": (None, [_vp]),
     "libgint_amd_plan_info": (None, [_vp, C.POINTER(C.c_uint64)]),
     "libgint_amd_queue_info": (None, [_vp, C.POINTER(C.c_uint64)]),

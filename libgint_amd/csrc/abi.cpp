@@ -129,6 +129,10 @@ void libgint_amd_set_debug(void *h, int flags) { E(h)->set_debug(flags); }
 void libgint_amd_plan_begin(void *h) { E(h)->plan_begin(); }
 void libgint_amd_plan_end(void *h) { E(h)->plan_end(); }
 void libgint_amd_plan_execute(void *h) { E(h)->plan_execute(); }
+int libgint_amd_plan_execute_noalloc(void *h, const double *P, const double *Pb, double *K, double *Kb, uint64_t n)
+{
+   return E(h)->plan_execute_noalloc(P, Pb, K, Kb, n);
+}
 void libgint_amd_plan_clear(void *h) { E(h)->plan_clear(); }
 void libgint_amd_plan_info(void *h, uint64_t out[6]) { E(h)->plan_info(out); }
 void libgint_amd_queue_info(void *h, uint64_t out[8]) { E(h)->queue_info(out); }

@@ -1258,10 +1258,10 @@ int Engine::plan_execute_noalloc(const double *Pa, const double *Pb, double *Ka,
    bool any = false;
    size_t ncp_max = 0;
    for (auto &D : plan_dev_) {
-      if (D.chunks_na.empty()) continue;
+      if (D.chunks.empty()) continue;   // a type with no recorded work
       any = true;
       const TypeInfo &ti = types_[D.type];
-      if (!D.na_fit)
+      if (!D.na_fit || D.chunks_na.empty())
          return err(-4, "a set quartet's primitive blocks exceed the no-allocation scratch (bytes of the largest)",
                     (long long)D.max_group_bytes);
       if (truncated_ && ti.plan.L > nder_) return err(-7, "the C0 table has too few derivative orders", ti.plan.L);
@@ -1295,7 +1295,7 @@ int Engine::plan_execute_noalloc(const double *Pa, const double *Pb, double *Ka,
    // the overflow word (it is only read back by the allocating path)
    if (launch_zero_u32(w.fixcnt, 2, stream_)) fatal("counter reset launch", 0);
    for (auto &D : plan_dev_) {
-      if (D.chunks_na.empty()) continue;
+      if (D.chunks.empty()) continue;
       const TypeInfo &ti = types_[D.type];
       const unsigned g = ti.cq_gws ? (unsigned)std::min<size_t>(SCRATCH_GRID, NOALLOC_GWS_BYTES / 8 /
                                                                                 ((size_t)GQS_PARTS * ti.cq_gws))
