@@ -82,6 +82,10 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--max-mem", type=int, default=65536, help="device budget, MiB")
+    ap.add_argument("--roof-stride", type=int, default=16,
+                    help="the serialised, instrumented roofline run takes every n-th set quartet")
+    ap.add_argument("--no-progress", dest="progress", action="store_false",
+                    help="no per-quartet-type progress lines (and no host wait per type) in the steps")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -146,16 +150,25 @@ def main():
     lib.libgint_amd_plan_info(h, info)
     log(f"set-level ingestion {t_ing:.1f}s, resident plan {info[0] / 2**30:.2f} GiB, {info[1]} quartet types, "
         f"{info[2]} chunks")
+    # debug bit 2: the engine waits for each quartet type's work and reports
+    # it (progress of long K-builds; one host wait per type, ~40 per step)
+    lib.libgint_amd_set_debug(h, 4 if a.progress else 0)
 
     def step(PP=P, out=K):
         lib.libgint_set_P(h, dp(PP), s.P_size)
         lib.libgint_amd_plan_execute(h)
         lib.libgint_get_K(h, dp(out))
 
+    # warm-up steps count the surviving slots (device counters, untimed)
+    lib.libgint_amd_enable_stats(h, 1)
+    lib.libgint_amd_reset_counters(h)
     for w in range(a.warmup):
         t = time.perf_counter()
         step()
         log(f"warmup {w} {time.perf_counter() - t:.3f}s")
+    stats = D.stats(lib, "libgint_", h)
+    slots = stats[1] // max(a.warmup, 1)
+    lib.libgint_amd_enable_stats(h, 0)
     lib.libgint_amd_synchronize(h)
     times = []
     for k in range(a.steps):
@@ -166,41 +179,53 @@ def main():
         times.append(time.perf_counter() - t)
         log(f"step {k} {times[-1]:.3f}s")
     tot = sum(times)
+    lib.libgint_amd_destroy_handle(h)
 
-    # one more step, instrumented and serialised: survivors, kernel times
-    lib.libgint_amd_enable_stats(h, 1)
-    lib.libgint_amd_enable_timing(h, 1)
-    lib.libgint_amd_set_debug(h, 2)
-    lib.libgint_amd_reset_counters(h)
+    # the roofline: a stride sample of the same workload as its own plan,
+    # executed once more with every kernel serialised on one stream and
+    # timed with HIP events around each launch
+    rs = a.roof_stride
+    hr, tr = D.setup_handle(lib, "libgint_", s, P=P, max_mem_mib=a.max_mem, c0=c0, timing=True, stats=True)
+    lib.libgint_amd_plan_begin(hr)
+    st.replay(tr, hr, 0, st.n_sets, a.stride * rs, True)
+    lib.libgint_amd_plan_end(hr)
+    lib.libgint_amd_set_debug(hr, 2)
+    lib.libgint_set_P(hr, dp(P), s.P_size)
+    lib.libgint_amd_plan_execute(hr)
+    lib.libgint_amd_synchronize(hr)   # first execution: buffers, code objects
+    lib.libgint_amd_reset_counters(hr)
     Ki = np.zeros(s.P_size)
     t = time.perf_counter()
-    step(out=Ki)
+    lib.libgint_set_P(hr, dp(P), s.P_size)
+    lib.libgint_amd_plan_execute(hr)
+    lib.libgint_get_K(hr, dp(Ki))
     t_ser = time.perf_counter() - t
-    stats = D.stats(lib, "libgint_", h)
+    rstats = D.stats(lib, "libgint_", hr)
     tm = (C.c_double * 12)()
-    lib.libgint_amd_timing(h, tm)
-    log(f"serialised instrumented step {t_ser:.3f}s; kernel ms fm_r {tm[0]:.0f} tq {tm[1]:.0f} gqs {tm[2]:.0f} "
-        f"fixup {tm[3]:.0f}; K equal to the timed steps' to {np.max(np.abs(Ki - K)):.2e}")
+    lib.libgint_amd_timing(hr, tm)
+    nsr = len(range(0, st.n_sets, a.stride * rs))
+    log(f"roofline sample (every {a.stride * rs}th set quartet, {nsr}): serialised instrumented step {t_ser:.3f}s; "
+        f"kernel ms fm_r {tm[0]:.0f} tq {tm[1]:.0f} gqs {tm[2]:.0f} fixup {tm[3]:.0f}")
     for cat, nm in ((0, "fm_r by type L"), (1, "tq by type L"), (2, "gqs by type L")):
         det = (C.c_double * 256)()
-        lib.libgint_amd_timing_detail(h, cat, det)
+        lib.libgint_amd_timing_detail(hr, cat, det)
         top = sorted(((det[i], i) for i in range(256) if det[i] > 0), reverse=True)[:12]
         log(f"{nm}: " + ", ".join(f"L{i} {ms:.0f}ms" for ms, i in top))
-    slots = stats[1]
-    value = slots / (tot / a.steps)
+    lib.libgint_amd_destroy_handle(hr)
+    value = slots / (tot / a.steps) if a.steps else None
     names = ["fm_r", "tq", "gqs", "fixup"]
     kms = [tm[i] for i in range(4)]
-    kfl = [float(stats[6]), tm[9], tm[10], 0.0]
+    kfl = [float(rstats[6]), tm[9], tm[10], 0.0]
     nl = [tm[4 + i] for i in range(4)]
     dom = int(np.argmax(kms))
     achieved = kfl[dom] / (kms[dom] * 1e-3) / 1e12 if kms[dom] > 0 else 0.0
     traffic, traffic_src = measured_traffic(names[dom], s.describe())
     roof = {"kernel": names[dom], "bound": "fp64-valu", "achieved": round(achieved, 4), "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 5), "traffic": traffic,
-            "traffic_source": traffic_src, "launches_per_step": nl[dom],
+            "traffic_source": traffic_src, "launches": nl[dom],
             "avg_launch_ms": round(kms[dom] / max(nl[dom], 1), 4),
-            "timing": "HIP events around each launch on its stream, all launches of the step serialised on one "
-                      "stream (an extra, instrumented step)",
+            "sample": f"every {a.stride * rs}th set quartet of the workload ({nsr}) as its own resident plan",
+            "timing": "HIP events around each launch on its stream, every launch serialised on one stream",
             "kernel_ms_serialised": {n: round(m, 2) for n, m in zip(names, kms)},
             "kernel_tflops": {n: round(f / (m * 1e-3) / 1e12, 4) if m > 0 else 0 for n, f, m in zip(names, kfl, kms)},
             "flops": "fm_r: the reference chain's Fm + VRR FLOPs of the kept (slot, class) pairs; tq, gqs: "
@@ -255,15 +280,15 @@ def main():
         log(f"max|dK| {err:.3e} on {check['sample']}; cpu baseline {cpu['sample']}")
 
     out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "quartets/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": round(1e3 * tot / a.steps, 3), "higher_is_better": True,
+        "metric": METRIC, "value": round(value, 1) if value else None, "unit": "quartets/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * tot / a.steps, 3) if a.steps else None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": s.describe(), "config_index": a.config, "atoms": s.natom, "nbf": s.nbf,
                    "cells": s.Ng, "set_quartets": n_sets, "stride": a.stride,
                    "primitive_quartets": st.n_prm if a.stride == 1 else None,
                    "contracted_quartets": st.n_qrtt if a.stride == 1 else None,
                    "surviving_slots_per_step": int(slots),
-                   "kbuild_s_per_step": round(tot / a.steps, 4),
+                   "kbuild_s_per_step": round(tot / a.steps, 4) if a.steps else None,
                    "abi_ingestion": abi, "set_level_ingestion_s": round(t_ing, 2),
                    "resident_plan_GiB": round(info[0] / 2**30, 3),
                    "path": "FP64, resident plan, fused Hermite cell sum per quartet type"},

@@ -130,7 +130,9 @@ void libgint_amd_timing_detail(void *handle, int cat, double out[256]);
 /* bit 0: route every surviving slot through the class-dependent screening
    path (a test of that path; results are unchanged); bit 1: run every
    kernel of a chunk on the handle's stream, one after the other (per-kernel
-   durations without overlap, for profiling; results are unchanged) */
+   durations without overlap, for profiling; results are unchanged); bit 2:
+   wait for the work of each quartet type and report its completion on
+   stderr (progress of long K-builds; results are unchanged) */
 void libgint_amd_set_debug(void *handle, int flags);
 /* resident plans: the calls between plan_begin and plan_end are kept on the
    device; plan_execute adds their exchange terms to K for the density of the

@@ -9,6 +9,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -1237,8 +1238,21 @@ void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
       kt.P[s] = d_P_[s];
       kt.K[s] = d_K_[s];
    }
-   for (auto &D : devs)
+   // debug bit 2: wait for each quartet type and report it on stderr
+   const bool progress = (debug_ & 4) != 0;
+   const auto t0 = std::chrono::steady_clock::now();
+   size_t nty = 0, done = 0;
+   for (auto &D : devs) nty += D.chunks.empty() ? 0 : 1;
+   for (auto &D : devs) {
+      if (D.chunks.empty()) continue;
       for (auto &c : D.chunks) run_chunk(types_[D.type], D, c, w, kt);
+      if (!progress) continue;
+      synchronize();
+      const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      std::fprintf(stderr, "libgint: quartet type %zu of %zu done (L %d, %zu set quartets, %zu chunks) at %.1f s\n",
+                   ++done, nty, types_[D.type].plan.L, D.n_groups, D.chunks.size(), sec);
+      std::fflush(stderr);
+   }
 }
 
 // The resident plan, like plan_execute, but into the caller's device P and
