@@ -65,6 +65,56 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
+def serving(a):
+    """the serving workload: independent periodic systems (20-200 atoms,
+    O 3s2p1d / H 2s1p, 27 cells) sending SCF-like K-build requests at
+    Poisson arrival times to one device-resident request loop under a
+    device budget; p50 / p99 latency and surviving slots per second"""
+    from libgint_amd import driver as D, load
+
+    lib = load()
+    t0 = time.time()
+    systems = D.serving_systems(n_systems=a.clients, seed=3)
+    clients = []
+    surv = []
+    for i, sy in enumerate(systems):
+        c = D.ServingClient(lib, sy, steps=10, seed=i)
+        # survivors of one K-build of this client (device counters, untimed)
+        lib.libgint_amd_enable_stats(c.h, 1)
+        lib.libgint_amd_plan_execute(c.h)
+        Kw = np.zeros(sy.P_size)
+        lib.libgint_get_K(c.h, D._ptr(Kw))
+        surv.append(D.stats(lib, "libgint_", c.h)[1])
+        lib.libgint_amd_enable_stats(c.h, 0)
+        clients.append(c)
+        log(f"client {i}: {sy.describe()}; set quartets {c.st.n_sets}, primitive quartets {c.st.n_prm}, "
+            f"surviving slots per K-build {surv[-1]}")
+    log(f"clients ready in {time.time() - t0:.1f}s")
+    done, st = D.serve(lib, clients, n_requests=a.requests, rate=a.rate, slots=a.slots, max_mem_mib=a.budget,
+                       seed=11)
+    ok = [d for d in done if d[3] == 0]
+    slots = sum(surv[d[0]] for d in ok)
+    wall = st[4]
+    lat = np.array([d[4] for d in ok])
+    log(f"served {len(ok)} of {len(done)} requests in {wall:.2f}s: p50 {st[1] * 1e3:.1f} ms, p99 {st[2] * 1e3:.1f} ms, "
+        f"mean {st[3] * 1e3:.1f} ms; peak resident {st[5] / 2**20:.0f} MiB of {a.budget}; evictions {st[6]:.0f}")
+    out = {
+        "metric": METRIC, "value": round(slots / wall, 1), "unit": "quartets/s", "n_gpus": 1, "steps": len(ok),
+        "warmup": 0, "ms_per_step": round(1e3 * float(np.mean(lat)), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "p50_ms": round(st[1] * 1e3, 3), "p99_ms": round(st[2] * 1e3, 3),
+        "config": {"workload": f"serving: {a.requests} K-build requests of {a.clients} periodic water-like systems "
+                               f"(20-200 atoms, 27 cells), Poisson arrivals at {a.rate} req/s, {a.slots} slots, "
+                               f"{a.budget} MiB device budget",
+                   "config_index": 3, "requests": len(done), "served": len(ok), "wall_s": round(wall, 3),
+                   "peak_resident_MiB": round(st[5] / 2**20, 1), "evictions": int(st[6]), "admissions": int(st[7]),
+                   "path": "FP64, one resident plan per client, D and K resident on the device"},
+    }
+    print(json.dumps(out), flush=True)
+    for c in clients:
+        lib.libgint_amd_destroy_handle(c.h)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +134,13 @@ def main():
     ap.add_argument("--max-mem", type=int, default=65536, help="device budget, MiB")
     ap.add_argument("--roof-stride", type=int, default=16,
                     help="the serialised, instrumented roofline run takes every n-th set quartet")
+    ap.add_argument("--mixed", type=float, default=1e-7,
+                    help="mixed-precision threshold tau for the extra mixed steps (0: none)")
+    ap.add_argument("--clients", type=int, default=16, help="serving workload: systems")
+    ap.add_argument("--requests", type=int, default=400, help="serving workload: requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving workload: mean arrivals per second")
+    ap.add_argument("--slots", type=int, default=4, help="serving workload: concurrent requests")
+    ap.add_argument("--budget", type=int, default=7000, help="serving workload: device budget, MiB")
     ap.add_argument("--no-progress", dest="progress", action="store_false",
                     help="no per-quartet-type progress lines (and no host wait per type) in the steps")
     a = ap.parse_args()
@@ -106,6 +163,9 @@ def main():
 
     from libgint_amd import driver as D, load, read_c0
 
+    if a.config == 3:
+        serving(a)
+        return
     lib = load()
     c0 = read_c0(a.c0_file, 21) if a.c0_file else None
     t0 = time.time()
@@ -179,6 +239,42 @@ def main():
         times.append(time.perf_counter() - t)
         log(f"step {k} {times[-1]:.3f}s")
     tot = sum(times)
+
+    # mixed precision: set quartets whose calibrated bound x max|P| < tau
+    # take the single-precision matrix-core contraction; one FP64 calibration
+    # execution (untimed), then the same number of timed steps, K compared
+    # with the FP64 steps' K of the same density
+    mixed = None
+    if a.mixed > 0 and a.steps:
+        lib.libgint_amd_set_mixed_precision(h, a.mixed)
+        Kc = np.zeros(s.P_size)
+        t = time.perf_counter()
+        step(out=Kc)
+        log(f"mixed precision: calibration step {time.perf_counter() - t:.3f}s")
+        mtimes = []
+        Km = np.zeros(s.P_size)
+        for k in range(a.steps):
+            lib.libgint_amd_synchronize(h)
+            t = time.perf_counter()
+            step(out=Km)
+            lib.libgint_amd_synchronize(h)
+            mtimes.append(time.perf_counter() - t)
+            log(f"mixed step {k} {mtimes[-1]:.3f}s")
+        mi = (C.c_double * 4)()
+        lib.libgint_amd_mixed_info(h, mi)
+        dk = float(np.max(np.abs(Km - K)))
+        mixed = {"tau": a.mixed, "s_per_step": round(sum(mtimes) / len(mtimes), 4),
+                 "speedup_vs_fp64": round((tot / a.steps) / (sum(mtimes) / len(mtimes)), 4),
+                 "max_abs_dK_vs_fp64": dk, "bar_Ha": 1e-8, "routed_set_quartets": int(mi[2]),
+                 "set_quartets": int(mi[3]),
+                 "path": "single-precision matrix-core contraction of the routed set quartets, the rest FP64"}
+        log(f"mixed precision tau {a.mixed}: {mixed['s_per_step']}s per step (x{mixed['speedup_vs_fp64']}), "
+            f"routed {int(mi[2])} of {int(mi[3])} set quartets, max|dK| vs FP64 {dk:.3e}")
+        lib.libgint_amd_set_mixed_precision(h, 0.0)
+    tu = (C.c_double * 5)()
+    lib.libgint_amd_tune_info(h, tu)
+    tune = {"classes_templated": int(tu[1]), "classes_workgroup": int(tu[2]), "classes_matrix_core": int(tu[3]),
+            "seconds": round(tu[4], 2)}
     lib.libgint_amd_destroy_handle(h)
 
     # the roofline: a stride sample of the same workload as its own plan,
@@ -203,20 +299,22 @@ def main():
     rstats = D.stats(lib, "libgint_", hr)
     tm = (C.c_double * 12)()
     lib.libgint_amd_timing(hr, tm)
+    te = (C.c_double * 3)()
+    lib.libgint_amd_timing_eco(hr, te)
     nsr = len(range(0, st.n_sets, a.stride * rs))
     log(f"roofline sample (every {a.stride * rs}th set quartet, {nsr}): serialised instrumented step {t_ser:.3f}s; "
-        f"kernel ms fm_r {tm[0]:.0f} tq {tm[1]:.0f} gqs {tm[2]:.0f} fixup {tm[3]:.0f}")
-    for cat, nm in ((0, "fm_r by type L"), (1, "tq by type L"), (2, "gqs by type L")):
+        f"kernel ms fm_r {tm[0]:.0f} tq {tm[1]:.0f} gqs {tm[2]:.0f} fixup {tm[3]:.0f} eco {te[0]:.0f}")
+    for cat, nm in ((0, "fm_r by type L"), (1, "tq by type L"), (2, "gqs by type L"), (4, "eco by type L")):
         det = (C.c_double * 256)()
         lib.libgint_amd_timing_detail(hr, cat, det)
         top = sorted(((det[i], i) for i in range(256) if det[i] > 0), reverse=True)[:12]
         log(f"{nm}: " + ", ".join(f"L{i} {ms:.0f}ms" for ms, i in top))
     lib.libgint_amd_destroy_handle(hr)
     value = slots / (tot / a.steps) if a.steps else None
-    names = ["fm_r", "tq", "gqs", "fixup"]
-    kms = [tm[i] for i in range(4)]
-    kfl = [float(rstats[6]), tm[9], tm[10], 0.0]
-    nl = [tm[4 + i] for i in range(4)]
+    names = ["fm_r", "tq", "gqs", "fixup", "eco"]
+    kms = [tm[i] for i in range(4)] + [te[0]]
+    kfl = [float(rstats[6]), tm[9], tm[10], 0.0, te[2]]
+    nl = [tm[4 + i] for i in range(4)] + [te[1]]
     dom = int(np.argmax(kms))
     achieved = kfl[dom] / (kms[dom] * 1e-3) / 1e12 if kms[dom] > 0 else 0.0
     traffic, traffic_src = measured_traffic(names[dom], s.describe())
@@ -228,8 +326,10 @@ def main():
             "timing": "HIP events around each launch on its stream, every launch serialised on one stream",
             "kernel_ms_serialised": {n: round(m, 2) for n, m in zip(names, kms)},
             "kernel_tflops": {n: round(f / (m * 1e-3) / 1e12, 4) if m > 0 else 0 for n, f, m in zip(names, kfl, kms)},
-            "flops": "fm_r: the reference chain's Fm + VRR FLOPs of the kept (slot, class) pairs; tq, gqs: "
-                     "contraction 2 n_prm ncp + HRR + SPH + 9 Ns per contracted quartet"}
+            "flops": "fm_r: the reference chain's Fm + VRR FLOPs of the kept (slot, class) pairs; tq: "
+                     "contraction 2 n_prm ncp + HRR + SPH + 9 Ns per contracted quartet; eco (matrix cores): the "
+                     "contraction 2 n_prm ncp of the classes with many contractions; gqs: the rest of those "
+                     "classes and the contraction of the others"}
 
     # max|dK| and the CPU baseline: a strided sample of the same stream,
     # through the product (stream mode) and the FP64 oracle on all cores
@@ -292,7 +392,7 @@ def main():
                    "abi_ingestion": abi, "set_level_ingestion_s": round(t_ing, 2),
                    "resident_plan_GiB": round(info[0] / 2**30, 3),
                    "path": "FP64, resident plan, fused Hermite cell sum per quartet type"},
-        "max_dK": check, "roofline": roof, "cpu_baseline": cpu,
+        "max_dK": check, "mixed_precision": mixed, "autotune": tune, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -123,9 +123,14 @@ void libgint_amd_enable_timing(void *handle, int on);
    out[9], out[10] algorithmic FLOPs of tq and gqs (out[8] and out[11] = 0:
    the fm_r FLOPs are libgint_amd_stats out[6]) */
 void libgint_amd_timing(void *handle, double out[12]);
+/* the matrix-core contraction of the classes with many contractions (done
+   before the per-class kernel, which then reads its result): out[0] ms,
+   [1] launches, [2] algorithmic FLOPs (2 n_prm ncp per contraction) */
+void libgint_amd_timing_eco(void *handle, double out[3]);
 void libgint_amd_reset_counters(void *handle);
 /* ms per launch key of one category of libgint_amd_timing (0 fm_r, 1 tq,
-   2 gqs) by the quartet type's total angular momentum */
+   2 gqs, 3 fix-up, 4 matrix-core contraction) by the quartet type's total
+   angular momentum */
 void libgint_amd_timing_detail(void *handle, int cat, double out[256]);
 /* bit 0: route every surviving slot through the class-dependent screening
    path (a test of that path; results are unchanged); bit 1: run every
@@ -134,6 +139,42 @@ void libgint_amd_timing_detail(void *handle, int cat, double out[256]);
    wait for the work of each quartet type and report its completion on
    stderr (progress of long K-builds; results are unchanged) */
 void libgint_amd_set_debug(void *handle, int flags);
+/* libgint_amd_plan_execute replays the plan's launches (every chunk's
+   kernels on the handle's stream and its side streams) as one HIP graph:
+   captured at the first execution, replayed at the next ones while the
+   buffers, P / K arrays, potential and debug flags it was captured with are
+   unchanged (else captured again).  Executions with timing, statistics or
+   debug bits 1-2 on, and handles of an OpenMP team of more than one thread
+   (whose other threads make allocating and synchronous calls while one
+   would capture), run eagerly.  on = 0 turns the graph off (default on). */
+void libgint_amd_set_graph(void *handle, int on);
+/* out[0] graphs on, [1] captures, [2] replays, [3] a graph is held */
+void libgint_amd_graph_info(void *handle, uint64_t out[4]);
+/* Autotuner (default on): at the first libgint_amd_plan_execute of a plan,
+   each class of each quartet type gets the fastest of its schedules, timed
+   on the type's first set quartets (into a scratch K): the templated
+   one-wavefront kernel, the workgroup kernel with its own contraction, or
+   the matrix-core contraction followed by the workgroup kernel.  Decisions
+   are kept per (class, contractions, primitive quartets per row in powers
+   of two) for the process and reused.  on = 0 keeps the built-in choice. */
+void libgint_amd_set_autotune(void *handle, int on);
+/* Mixed precision for resident plans (tau = 0, the default: FP64 only).
+   The first execution after plan_end (or after tau is first set) runs in
+   FP64 and records each set quartet's largest primitive block element;
+   later executions send a set quartet whose bound -- that element times
+   its largest contraction coefficients (one per centre) times its
+   primitive quartets times max |P| of the density -- is below tau to the
+   single-precision matrix-core contraction (each batch of 16 primitive
+   quartets summed in single precision, batches, HRR, spherical transform
+   and K in double precision).  The Boys function, the Hermite recursion
+   and the other classes stay FP64. */
+void libgint_amd_set_mixed_precision(void *handle, double tau);
+/* out[0] tau, [1] calibrated, [2] set quartets routed to single precision
+   by the last execution, [3] set quartets of the plan */
+void libgint_amd_mixed_info(void *handle, double out[4]);
+/* out[0] the plan is tuned, [1..3] classes given the templated kernel /
+   the workgroup kernel / the matrix-core contraction, [4] seconds spent */
+void libgint_amd_tune_info(void *handle, double out[5]);
 /* resident plans: the calls between plan_begin and plan_end are kept on the
    device; plan_execute adds their exchange terms to K for the density of the
    last set_P (between set_P and get_K, in place of the calls) */
@@ -194,6 +235,35 @@ int libgint_amd_queue_types(void *handle, uint64_t *out, int max_types);
 void libgint_amd_add_set_blocks(void *handle, double symm_fac, int ld_ac, int ld_ad, int ld_bc, int ld_bd,
                                 unsigned int off_ac, unsigned int off_ad, unsigned int off_bc, unsigned int off_bd,
                                 bool Tac, bool Tad, bool Tbc, bool Tbd);
+/* Serving: K-builds of many clients (independent SCF runs) through one
+   device-resident request loop.  A client is a handle (libgint_init done)
+   whose call stream was recorded as a resident plan (plan_begin / plan_end);
+   add_client takes it over (its own stream from then on; do not use the
+   handle otherwise while the server runs) and keeps its density and
+   exchange matrix (P_size doubles each) on the device between requests.
+   submit queues one K-build: P is read and K overwritten (host arrays of
+   P_size doubles, kept valid until wait returns); requests run in arrival
+   order on `slots` worker threads as clients become idle and fit the device
+   budget (plan records, work buffers, tables, P, K of the resident clients
+   <= max_mem_MiB; idle clients are evicted least recently used first, their
+   plans kept on the host), so requests that arrive while others run overlap
+   with them on the GPU.  wait blocks until a request is done: out[0] latency
+   (submit to K on the host), [1] queueing, [2] execution, seconds; returns 0,
+   or -1 when the client can never fit the budget.  stats: out[0] requests
+   done, [1] p50 and [2] p99 latency, [3] mean latency, [4] first arrival to
+   last completion (s), [5] peak resident bytes, [6] evictions, [7]
+   admissions.  destroy waits for the workers; the client handles remain the
+   caller's. */
+void *libgint_amd_server_create(int max_mem_MiB, int slots);
+int libgint_amd_server_add_client(void *server, void *handle, int P_size);
+int64_t libgint_amd_server_submit(void *server, int client, const double *P, double *K);
+int libgint_amd_server_wait(void *server, int64_t request, double out[3]);
+void libgint_amd_server_stats(void *server, double out[8]);
+void libgint_amd_server_destroy(void *server);
+/* device bytes the budget counts for a client (measured after its first
+   request, estimated before), and a new budget (MiB) */
+int64_t libgint_amd_server_client_bytes(void *server, int client);
+void libgint_amd_server_set_budget(void *server, int max_mem_MiB);
 /* reads CP2K's truncated-Coulomb coefficient file (the T_C_G_DATA
    t_c_g.dat) as reference src/t_c_g0_n.cpp:29-51 (read_c0) does: 207
    patches of 22 x 105 white-space separated numbers, patch after patch.
@@ -208,6 +278,11 @@ int libgint_amd_read_c0(const char *path, int nder, double *C0, int capacity, in
    patch per line, 17 significant digits so that reading it back is exact;
    returns 0, -1 on an I/O error, -2 on a bad argument */
 int libgint_amd_write_c0(const char *path, const double *C0, int ld_C0);
+/* D (16 x 16, row-major) = A (16 x 4) B (4 x 16) on the matrix-core tile
+   the contraction kernel uses (prec 0: v_mfma_f64_16x16x4f64, prec 1:
+   v_mfma_f32_16x16x4f32, inputs rounded to float), with that kernel's lane
+   layout: a check of the layout against a host product; 0 or -1 */
+int libgint_amd_mfma_selftest(const double *A, const double *B, double *D, int prec);
 /* evaluates the screening arithmetic on the GPU for a bitwise comparison
    with the host: out[4i..4i+3] = exp(x), sqrt|x|, 1/x, x*y/(x+y) */
 int libgint_amd_math_selftest(const double *x, const double *y, int n, double *out);

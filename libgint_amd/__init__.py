@@ -56,9 +56,16 @@ ABI = {
     "libgint_amd_stats": (None, [_vp, C.POINTER(C.c_uint64)]),
     "libgint_amd_enable_timing": (None, [_vp, _i]),
     "libgint_amd_timing": (None, [_vp, _dp]),
+    "libgint_amd_timing_eco": (None, [_vp, _dp]),
     "libgint_amd_reset_counters": (None, [_vp]),
     "libgint_amd_timing_detail": (None, [_vp, _i, _dp]),
     "libgint_amd_set_debug": (None, [_vp, _i]),
+    "libgint_amd_set_graph": (None, [_vp, _i]),
+    "libgint_amd_set_autotune": (None, [_vp, _i]),
+    "libgint_amd_set_mixed_precision": (None, [_vp, C.c_double]),
+    "libgint_amd_mixed_info": (None, [_vp, _dp]),
+    "libgint_amd_tune_info": (None, [_vp, _dp]),
+    "libgint_amd_graph_info": (None, [_vp, C.POINTER(C.c_uint64)]),
     "libgint_amd_plan_begin": (None, [_vp]),
     "libgint_amd_plan_end": (None, [_vp]),
     "libgint_amd_plan_execute": (None, [_vp]),
@@ -68,8 +75,17 @@ ABI = {
     "libgint_amd_queue_info": (None, [_vp, C.POINTER(C.c_uint64)]),
     "libgint_amd_queue_types": (_i, [_vp, C.POINTER(C.c_uint64), _i]),
     "libgint_amd_add_set_blocks": (None, [_vp, C.c_double] + [_i] * 4 + [_u] * 4 + [_b] * 4),
+    "libgint_amd_mfma_selftest": (_i, [_dp, _dp, _dp, _i]),
     "libgint_amd_math_selftest": (_i, [_dp, _dp, _i, _dp]),
     "libgint_amd_math_host": (None, [_dp, _dp, _i, _dp]),
+    "libgint_amd_server_create": (_vp, [_i, _i]),
+    "libgint_amd_server_add_client": (_i, [_vp, _vp, _i]),
+    "libgint_amd_server_submit": (C.c_int64, [_vp, _i, _dp, _dp]),
+    "libgint_amd_server_wait": (_i, [_vp, C.c_int64, _dp]),
+    "libgint_amd_server_stats": (None, [_vp, _dp]),
+    "libgint_amd_server_destroy": (None, [_vp]),
+    "libgint_amd_server_client_bytes": (C.c_int64, [_vp, _i]),
+    "libgint_amd_server_set_budget": (None, [_vp, _i]),
     "libgint_amd_read_c0": (_i, [C.c_char_p, _i, _dp, _i, C.POINTER(_i)]),
     "libgint_amd_write_c0": (_i, [C.c_char_p, _dp, _i]),
 }

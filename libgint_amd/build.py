@@ -23,8 +23,8 @@ TCG_BIN = os.path.join(HERE, "data", "tcg_c0.bin")
 TCG_SHA256 = None
 REF_SRC = "/root/reference/src"
 
-HOST_SRC = ["engine.cpp", "abi.cpp", "plan.cpp", "boys_table.cpp", "c0_file.cpp"]
-DEV_SRC = ["kernels.hip", "kernels_tq.hip"]
+HOST_SRC = ["engine.cpp", "abi.cpp", "plan.cpp", "boys_table.cpp", "c0_file.cpp", "server.cpp"]
+DEV_SRC = ["kernels.hip", "kernels_tq.hip", "kernels_eco.hip"]
 # kernels_tq_part.hip is built once per part of the templated class kernels
 TQ_PARTS = 8
 HIPFLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",

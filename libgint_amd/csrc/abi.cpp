@@ -123,9 +123,22 @@ void libgint_amd_timing(void *h, double out[12])
       out[8 + i] = t.flops[i];
    }
 }
+void libgint_amd_timing_eco(void *h, double out[3])
+{
+   const gint::Timing t = E(h)->timing();
+   out[0] = t.ms[4];
+   out[1] = (double)t.launches[4];
+   out[2] = t.flops[4];
+}
 void libgint_amd_reset_counters(void *h) { E(h)->reset_counters(); }
 void libgint_amd_timing_detail(void *h, int cat, double out[256]) { E(h)->timing_detail(cat, out); }
 void libgint_amd_set_debug(void *h, int flags) { E(h)->set_debug(flags); }
+void libgint_amd_set_graph(void *h, int on) { E(h)->set_graph(on != 0); }
+void libgint_amd_set_autotune(void *h, int on) { E(h)->set_autotune(on != 0); }
+void libgint_amd_set_mixed_precision(void *h, double tau) { E(h)->set_mixed_precision(tau); }
+void libgint_amd_mixed_info(void *h, double out[4]) { E(h)->mixed_info(out); }
+void libgint_amd_tune_info(void *h, double out[5]) { E(h)->tune_info(out); }
+void libgint_amd_graph_info(void *h, uint64_t out[4]) { E(h)->graph_info(out); }
 void libgint_amd_plan_begin(void *h) { E(h)->plan_begin(); }
 void libgint_amd_plan_end(void *h) { E(h)->plan_end(); }
 void libgint_amd_plan_execute(void *h) { E(h)->plan_execute(); }
@@ -137,6 +150,10 @@ void libgint_amd_plan_clear(void *h) { E(h)->plan_clear(); }
 void libgint_amd_plan_info(void *h, uint64_t out[6]) { E(h)->plan_info(out); }
 void libgint_amd_queue_info(void *h, uint64_t out[8]) { E(h)->queue_info(out); }
 int libgint_amd_queue_types(void *h, uint64_t *out, int max_types) { return E(h)->queue_types(out, max_types); }
+int libgint_amd_mfma_selftest(const double *A, const double *B, double *D, int prec)
+{
+   return gint::mfma_selftest(A, B, D, prec);
+}
 int libgint_amd_math_selftest(const double *x, const double *y, int n, double *out)
 {
    return gint::device_math_selftest(x, y, n, out);

@@ -237,6 +237,25 @@ struct BatchDev {
    uint32_t fix_cap;
    int force_fix;             // route every surviving slot through the fix-up path (test hook)
    double *fix_ws;            // fix-up workspace, ncp doubles per entry
+   double *eco;               // contraction buffer: eco_row doubles per row of the chunk (k_eco)
+   size_t eco_row;
+   double *gmag;              // calibration: max |block element| per set quartet (nullable; absolute group index)
+   const uint32_t *gflag;     // mixed precision: 1 = single-precision contraction for the set quartet (nullable)
+};
+
+// One work item of the matrix-core contraction (k_eco) of a row: class k,
+// column tiles jt0 .. jt0 + 3 (16 columns each), contractions qb * 64 ..
+struct EcoItem {
+   uint16_t k, jt0, qb, pad;
+};
+
+struct EcoArgs {
+   const EcoItem *items;      // the items of one row (the same for every row of the type)
+   int n_items;
+   const int *srcmaps;        // the type's class column maps (make_cq_plan)
+   const int *src_off;        // [MAXTC] first entry of class k's map
+   const int *eco_off;        // [MAXTC] class k's offset in a row of the contraction buffer
+   size_t row_doubles;        // contraction buffer doubles per row
 };
 
 // the K layout of class k of group g
@@ -304,6 +323,8 @@ struct CqLayout {
    int skip;        // the class runs in its own templated kernel instead
    int part;        // which of the GQS_PARTS concurrent launches takes the class
    int bytes;
+   int eco;         // contracted on the matrix cores (k_eco): k_gqs reads the contraction buffer
+   int eco_off;     // the class's offset in a row of the contraction buffer
 };
 
 // address of the no-allocation scratch array on the current device, looked
@@ -324,12 +345,33 @@ int launch_tq(int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev 
 // per-class LDS plans and block maps of a type for the quartet kernel
 void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::vector<CqLayout> &lays,
                   std::vector<int> &srcmaps, int *lds_bytes, int *gws_stride);
+// the k_gqs classes of a type over its GQS_PARTS launches, balanced by cost
+void assign_parts(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::vector<CqLayout> &lays);
 // the classes of a type that k_gqs takes are split into this many launches,
 // balanced by their estimated cost, which run concurrently on separate streams
 constexpr int GQS_PARTS = 4;
+// npc: the classes of the type that this part takes (work items are
+// (set quartet, class) pairs)
+constexpr unsigned GQS_MAX_GRID = 8192;
 int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lays, const int *d_srcmaps, int lds_bytes,
-               int gws_stride, int part, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s,
+               int gws_stride, int part, int npc, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s,
                void *stream);
+// contraction of primitive blocks on the matrix cores for the classes with
+// CqLayout::eco (kernels_eco.hip): k_gqs classes with at least this many
+// contractions
+constexpr int ECO_MIN_NQ = 8;
+// classes with at least this many contractions have a place in the
+// contraction buffer (the autotuner may move them to the matrix cores)
+constexpr int ECO_CAND_NQ = 4;
+int launch_eco(const TypePlan &tp, const BatchDev &b, const EcoArgs &e, void *stream);
+// mixed precision: max |P| (*out, zeroed first) and the route of each set
+// quartet of a chunk from its calibrated magnitude
+int launch_absmax(const double *P, size_t n, unsigned long long *out, bool zero, void *stream);
+int launch_mixed_flags(const TypePlan &tp, const BatchDev &b, const double *gmag, const unsigned long long *pmax,
+                       double tau, uint32_t *gflag, unsigned long long *count, void *stream);
+// D (16 x 16, row-major) = A (16 x 4) B (4 x 16) through the matrix-core
+// tile the contraction uses, with its lane layout; 0 or -1
+int mfma_selftest(const double *A, const double *B, double *D, int prec);
 // LDS bytes of the fused kernel for a type (0 if it does not fit)
 size_t fm_r_lds_bytes(const TypePlan &tp);
 

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -81,9 +82,11 @@ void TypeQueue::release()
 
 void TypeDev::release()
 {
-   void *p[7] = {rows, groups, prm_ip, prm_row, ksg, ksx, kss};
+   void *p[9] = {rows, groups, prm_ip, prm_row, ksg, ksx, kss, gmag, gflag};
    for (void *x : p)
       if (x) (void)hipFree(x);
+   gmag = nullptr;
+   gflag = nullptr;
    rows = nullptr;
    groups = nullptr;
    prm_ip = prm_row = nullptr;
@@ -135,6 +138,8 @@ Engine::Engine()
 Engine::~Engine()
 {
    if (stream_) (void)hipStreamSynchronize((hipStream_t)stream_);
+   drop_graph();
+   if (own_stream_ && stream_) (void)hipStreamDestroy((hipStream_t)stream_);
    for (auto &d : stream_dev_) d.release();
    for (auto &d : plan_dev_) d.release();
    for (auto &ti : types_) {
@@ -144,9 +149,11 @@ Engine::~Engine()
       if (ti.d_cps) (void)hipFree(ti.d_cps);
       if (ti.d_lays) (void)hipFree(ti.d_lays);
       if (ti.d_srcmaps) (void)hipFree(ti.d_srcmaps);
+      if (ti.d_eco_info) (void)hipFree(ti.d_eco_info);
+      if (ti.d_eco_items) (void)hipFree(ti.d_eco_items);
    }
    void *bufs[] = {d_data_, d_C0_, d_grid_, d_bw_, d_ftab_, d_cell_, d_neighs_, d_work_,
-                   d_fix_, d_fix_ws_, d_scratch_, d_stats_, d_fixcnt_, d_out_};
+                   d_fix_, d_fix_ws_, d_scratch_, d_stats_, d_fixcnt_, d_out_, d_mix_};
    for (void *b : bufs)
       if (b) (void)hipFree(b);
    for (void *e : ev_pool_) (void)hipEventDestroy((hipEvent_t)e);
@@ -260,6 +267,27 @@ int Engine::set_sig(int i)
    return s.sig;
 }
 
+// The matrix-core contraction of a type: its classes with CqLayout::eco,
+// one work item per 64 columns x 64 contractions of each, and the row
+// length of the contraction buffer.
+static void make_eco_items(TypeInfo &ti)
+{
+   const TypePlan &tp = ti.plan;
+   ti.eco_items.clear();
+   ti.eco_row = 0;
+   ti.eco_info.assign(2 * MAXTC, 0);
+   for (int k = 0; k < tp.nclass; k++) {
+      const CqLayout &y = ti.lays[k];
+      ti.eco_info[k] = y.o_src;
+      ti.eco_info[MAXTC + k] = y.eco_off;
+      const int cols = tp.cls_ncp[k], nq = tp.cls_nl[k];
+      if (y.eco_off >= 0) ti.eco_row = std::max(ti.eco_row, (size_t)y.eco_off + (size_t)nq * cols);
+      if (!y.eco) continue;
+      for (int jt0 = 0; jt0 * 16 < cols; jt0 += 4)
+         for (int qb = 0; qb * 64 < nq; qb++) ti.eco_items.push_back(EcoItem{(uint16_t)k, (uint16_t)jt0, (uint16_t)qb, 0});
+   }
+}
+
 int Engine::type_of(int i, int j, int k, int l)
 {
    const std::array<int, 4> sets = {i, j, k, l};
@@ -290,6 +318,7 @@ int Engine::type_of(int i, int j, int k, int l)
          ti.cplans.push_back(make_plan(code / 64, (code / 16) % 4, (code / 4) % 4, code % 4));
       }
       make_cq_plan(ti.plan, ti.cplans, ti.lays, ti.srcmaps, &ti.cq_lds, &ti.cq_gws);
+      make_eco_items(ti);
       types_.push_back(ti);
       q_.emplace_back();
       it = type_ids_.emplace(sig, (int)types_.size() - 1).first;
@@ -755,9 +784,12 @@ void Engine::plan_end()
    if (cur_type_ >= 0) fatal("plan_end with an open set: call add_set first", cur_type_);
    recording_ = false;
    synchronize();
+   drop_graph();
    for (auto &d : plan_dev_) d.release();
    plan_dev_.clear();
    upload(q_, plan_dev_, true);
+   plan_tuned_ = false;
+   mixed_calibrated_ = false;
    // tables and basis data resident now, so that an execution needs no
    // allocation (plan_execute_noalloc)
    ensure_tables();
@@ -772,12 +804,60 @@ void Engine::plan_execute()
    if (recording_) fatal("plan_execute while recording", 0);
    if (!d_P_[0]) fatal("plan_execute before set_P", 0);
    dispatch_all();
-   execute(plan_dev_, plan_max_index_);
+   RunBufs w{};
+   KTargets kt{};
+   if (!prepare_exec(plan_dev_, plan_max_index_, w, kt, FP_size_)) return;
+   // (not while an integral-output array is set: its classes write there)
+   if (autotune_on_ && !plan_tuned_ && !d_out_) autotune(w, kt);
+   mixed_mode_ = 0;
+   if (mixed_tau_ > 0.0 && !d_out_) {
+      ensure_mixed();
+      mixed_mode_ = mixed_calibrated_ ? 2 : 1;
+   }
+   // per-launch events, device counters and host waits stay eager; so does
+   // a handle of a team of several threads: while one thread captured its
+   // stream, the others make allocating and synchronous calls of the
+   // protocol (plan_end, set_P, get_K) that a capture in progress forbids
+   if (graph_on_ && nomp_ == 1 && !timing_on_ && !stats_on_ && (debug_ & 6) == 0)
+      replay_graph(w, kt);
+   else
+      launch_all(plan_dev_, w, kt);
+   if (mixed_mode_ == 1) mixed_calibrated_ = true;
+   mixed_mode_ = 0;
+}
+
+// device arrays of the mixed-precision route (plans)
+void Engine::ensure_mixed()
+{
+   if (!d_mix_) {
+      HIPCHECK(hipMalloc((void **)&d_mix_, 2 * sizeof(unsigned long long)));
+      HIPCHECK(hipMemset(d_mix_, 0, 2 * sizeof(unsigned long long)));
+   }
+   for (auto &D : plan_dev_) {
+      if (!D.n_groups || D.gmag) continue;
+      HIPCHECK(hipMalloc((void **)&D.gmag, sizeof(double) * D.n_groups));
+      HIPCHECK(hipMalloc((void **)&D.gflag, sizeof(uint32_t) * D.n_groups));
+      mixed_calibrated_ = false;
+   }
+}
+
+void Engine::mixed_info(double out[4])
+{
+   synchronize();
+   out[0] = mixed_tau_;
+   out[1] = mixed_calibrated_ ? 1.0 : 0.0;
+   unsigned long long c[2] = {0, 0};
+   if (d_mix_) HIPCHECK(hipMemcpy(c, d_mix_, sizeof c, hipMemcpyDeviceToHost));
+   out[2] = (double)c[1];
+   uint64_t ng = 0;
+   for (auto &D : plan_dev_) ng += D.n_groups;
+   out[3] = (double)ng;
 }
 
 void Engine::plan_clear()
 {
    synchronize();
+   drop_graph();
    for (auto &d : plan_dev_) d.release();
    plan_dev_.clear();
 }
@@ -942,6 +1022,13 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
          HIPCHECK(hipMemcpy(ti.d_lays, ti.lays.data(), sizeof(CqLayout) * ti.lays.size(), hipMemcpyHostToDevice));
          HIPCHECK(hipMalloc((void **)&ti.d_srcmaps, sizeof(int) * ti.srcmaps.size()));
          HIPCHECK(hipMemcpy(ti.d_srcmaps, ti.srcmaps.data(), sizeof(int) * ti.srcmaps.size(), hipMemcpyHostToDevice));
+         HIPCHECK(hipMalloc((void **)&ti.d_eco_info, sizeof(int) * ti.eco_info.size()));
+         HIPCHECK(hipMemcpy(ti.d_eco_info, ti.eco_info.data(), sizeof(int) * ti.eco_info.size(), hipMemcpyHostToDevice));
+         if (!ti.eco_items.empty()) {
+            HIPCHECK(hipMalloc((void **)&ti.d_eco_items, sizeof(EcoItem) * ti.eco_items.size()));
+            HIPCHECK(hipMemcpy(ti.d_eco_items, ti.eco_items.data(), sizeof(EcoItem) * ti.eco_items.size(),
+                               hipMemcpyHostToDevice));
+         }
       }
       dev_put(D.rows, D.cap[0], Q.rows);
       dev_put(D.groups, D.cap[1], Q.groups);
@@ -979,7 +1066,9 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
          const GroupRec &g = Q.groups[gi];
          size_t gp = 0;
          for (uint32_t r = g.row0; r < g.row1; r++) gp += Q.rows[r].n_prm;
-         gbytes[gi] = gp * sizeof(double) * (size_t)tp.ncp;
+         // its primitive blocks and, for the matrix-core classes, its rows of
+         // the contraction buffer (behind the chunk's blocks in the work buffer)
+         gbytes[gi] = gp * sizeof(double) * (size_t)tp.ncp + (size_t)(g.row1 - g.row0) * ti.eco_row * sizeof(double);
          D.max_group_bytes = std::max(D.max_group_bytes, gbytes[gi]);
       }
       auto make_chunks = [&](size_t wbytes, std::vector<Chunk> &out, bool flops) {
@@ -988,8 +1077,10 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
          for (size_t gi = 0; gi < D.n_groups; gi++) {
             const GroupRec &g = Q.groups[gi];
             const size_t gb = gbytes[gi];
-            const size_t gp = gb / (sizeof(double) * (size_t)tp.ncp);
+            const size_t gp = (gb - (size_t)(g.row1 - g.row0) * ti.eco_row * sizeof(double)) /
+                              (sizeof(double) * (size_t)tp.ncp);
             if (bytes + gb > wbytes && c.g1 > c.g0) {
+               if (flops) D.max_chunk_bytes = std::max(D.max_chunk_bytes, bytes);
                out.push_back(c);
                c = Chunk{};
                c.g0 = c.g1 = (uint32_t)gi;
@@ -1011,13 +1102,24 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
             for (int k = 0; k < tp.nclass; k++) {
                const double nq = ks_count(gks(gi, k).flags);
                const ClassPlan &cp = ti.cplans[k];
-               const double f = nq * (2.0 * (double)gp * cp.ncp + (double)(g.row1 - g.row0) * cp.hrr_flops +
-                                      cp.sph_flops + 9.0 * cp.Ns);
-               (ti.lays[k].skip ? c.tq_flops : c.k3_flops) += f;
+               const double contr = nq * 2.0 * (double)gp * cp.ncp;
+               const double f = nq * ((double)(g.row1 - g.row0) * cp.hrr_flops + cp.sph_flops + 9.0 * cp.Ns);
+               if (ti.lays[k].skip) {
+                  c.tq_flops += contr + f;
+               } else if (ti.lays[k].eco) {
+                  c.eco_flops += contr;
+                  c.k3_flops += f;
+               } else {
+                  c.k3_flops += contr + f;
+               }
             }
          }
-         if (c.g1 > c.g0) out.push_back(c);
+         if (c.g1 > c.g0) {
+            if (flops) D.max_chunk_bytes = std::max(D.max_chunk_bytes, bytes);
+            out.push_back(c);
+         }
       };
+      D.max_chunk_bytes = 0;
       make_chunks(work, D.chunks, true);
       D.chunks_na.clear();
       D.na_fit = D.max_group_bytes <= NOALLOC_AC0_BYTES;
@@ -1027,10 +1129,12 @@ void Engine::upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool
    }
 }
 
-void Engine::ensure_work(size_t ncp_max, size_t max_prm_chunk, size_t max_group_bytes)
+void Engine::ensure_work(size_t ncp_max, size_t max_prm_chunk, size_t max_chunk_bytes)
 {
-   const size_t want = std::max(budget_bytes_ / 2, max_group_bytes);
-   if (!d_work_ || work_bytes_ != want) {
+   // as large as the largest chunk (at most half the budget, or one set
+   // quartet that is larger); grows, never shrinks (plan_evict frees it)
+   const size_t want = std::max<size_t>(max_chunk_bytes, 1 << 20);
+   if (!d_work_ || want > work_bytes_) {
       synchronize();
       if (d_work_) HIPCHECK(hipFree(d_work_));
       HIPCHECK(hipMalloc((void **)&d_work_, want));
@@ -1119,7 +1223,7 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c, co
    const TypePlan &tp = ti.plan;
    hipStream_t st = (hipStream_t)stream_;
    const DevTables t = dev_tables();
-   BatchDev b;
+   BatchDev b{};   // every field zero / null unless set below
    b.tp = ti.d_plan;
    b.e1 = ti.d_e1;
    b.e2 = ti.d_e2;
@@ -1144,6 +1248,10 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c, co
    b.fix_cap = w.fix_cap;
    b.force_fix = debug_ & 1;
    b.fix_ws = w.fix_ws;
+   b.eco = w.ac0 + (size_t)(c.p1 - c.p0) * tp.ncp;
+   b.eco_row = ti.eco_row;
+   b.gmag = mixed_mode_ == 1 ? td.gmag : nullptr;
+   b.gflag = mixed_mode_ == 2 ? td.gflag : nullptr;
    auto evb = [&](int cat, void *q) { if (w.count) ev_begin(cat, q); };
    auto eve = [&](int cat, void *q) { if (w.count) ev_end(cat, q); };
 
@@ -1157,6 +1265,22 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c, co
    rc = launch_fixup(tp, t, b, (debug_ & 1) ? w.fix_cap : std::min<uint32_t>(64u, w.fix_cap), st);
    if (rc) fatal("fix-up launch", rc);
    eve(3, st);
+   if (mixed_mode_ == 2 && launch_mixed_flags(tp, b, td.gmag, d_mix_, mixed_tau_, td.gflag, d_mix_ + 1, st))
+      fatal("mixed-precision route launch", 0);
+   if (!ti.eco_items.empty()) {
+      EcoArgs e{};
+      e.items = ti.d_eco_items;
+      e.n_items = (int)ti.eco_items.size();
+      e.srcmaps = ti.d_srcmaps;
+      e.src_off = ti.d_eco_info;
+      e.eco_off = ti.d_eco_info + MAXTC;
+      e.row_doubles = ti.eco_row;
+      evb(4, st);
+      ev_key_ = tp.L;
+      rc = launch_eco(tp, b, e, st);
+      if (rc) fatal("matrix-core contraction launch", rc);
+      eve(4, st);
+   }
    // debug bit 1: every consumer on the main stream, one after the other
    // (kernel durations without overlap, for profiling)
    const bool serial = (debug_ & 2) != 0;
@@ -1165,13 +1289,17 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c, co
    for (int i = 0; i < NSIDE; i++) HIPCHECK(hipStreamWaitEvent((hipStream_t)side_[i], (hipEvent_t)ev_join_[0], 0));
    for (int part = 0; part < GQS_PARTS; part++) {
       bool any = false;
-      for (int k = 0; k < tp.nclass; k++)
-         any = any || (td.class_used[k] && !ti.lays[k].skip && ti.lays[k].part == part);
+      int npc = 0;
+      for (int k = 0; k < tp.nclass; k++) {
+         if (ti.lays[k].skip || ti.lays[k].part != part) continue;
+         npc++;
+         any = any || td.class_used[k];
+      }
       if (!any) continue;
       void *q = (part == 0 || serial) ? (void *)st : side_[(part - 1) % NSIDE];
       evb(2, q);
       ev_key_ = tp.L;
-      rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, part, t, b, kt, w.gws, q);
+      rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, part, npc, t, b, kt, w.gws, q);
       if (rc) fatal("gqs launch", rc);
       eve(2, q);
    }
@@ -1192,13 +1320,17 @@ void Engine::run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c, co
    if (!w.count) return;
    timing_.flops[1] += c.tq_flops;
    timing_.flops[2] += c.k3_flops;
+   timing_.flops[4] += c.eco_flops;
    n_prmc_ += (uint64_t)(c.p1 - c.p0) * tp.nclass;
    n_slots_ += (uint64_t)(c.p1 - c.p0) * (uint64_t)t.Ng * tp.nclass;
 }
 
-void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
+// Device buffers and targets of an execution (allocating or growing what
+// the work needs; nothing of this is captured in a graph); false when there
+// is no work.
+bool Engine::prepare_exec(std::vector<TypeDev> &devs, uint64_t max_index, RunBufs &w, KTargets &kt, uint64_t n)
 {
-   if (max_index >= FP_size_) fatal("add_qrtt offsets reach beyond the P / K size of set_P", (long long)max_index);
+   if (max_index >= n) fatal("add_qrtt offsets reach beyond the P / K size of set_P", (long long)max_index);
    ensure_tables();
    upload_data();
    size_t ncp_max = 0, max_prm = 0, scr = 0, max_gb = 0;
@@ -1209,11 +1341,11 @@ void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
          fatal("the truncated-Coulomb table has fewer derivative orders than this quartet's total angular momentum",
                ti.plan.L);
       ncp_max = std::max(ncp_max, (size_t)ti.plan.ncp);
-      max_gb = std::max(max_gb, D.max_group_bytes);
+      max_gb = std::max(max_gb, D.max_chunk_bytes);
       for (auto &c : D.chunks) max_prm = std::max(max_prm, (size_t)(c.p1 - c.p0));
       scr = std::max(scr, (size_t)ti.cq_gws * SCRATCH_GRID * GQS_PARTS);
    }
-   if (ncp_max == 0) return;
+   if (ncp_max == 0) return false;
    ensure_work(ncp_max, max_prm, max_gb);
    if (scr > scratch_cap_) {
       synchronize();
@@ -1221,7 +1353,6 @@ void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
       HIPCHECK(hipMalloc((void **)&d_scratch_, scr * sizeof(double)));
       scratch_cap_ = scr;
    }
-   RunBufs w;
    w.ac0 = d_work_;
    w.fix = d_fix_;
    w.fixcnt = d_fixcnt_;
@@ -1230,13 +1361,28 @@ void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
    w.stats = stats_on_ ? d_stats_ : nullptr;
    w.gws = Scratch{d_scratch_, scratch_cap_, SCRATCH_GRID};
    w.count = true;
-   KTargets kt;
    kt.nspin = nspin_;
    kt.hf_fac = hf_fac_;
    kt.out = d_out_;
    for (int s = 0; s < 2; s++) {
       kt.P[s] = d_P_[s];
       kt.K[s] = d_K_[s];
+   }
+   return true;
+}
+
+void Engine::launch_all(std::vector<TypeDev> &devs, const RunBufs &w, const KTargets &kt)
+{
+   if (mixed_mode_ == 1) {
+      // calibration: every set quartet's magnitude from 0
+      for (auto &D : devs)
+         if (D.gmag && launch_zero_u32((uint32_t *)D.gmag, (int)(2 * D.n_groups), stream_))
+            fatal("calibration reset launch", 0);
+   } else if (mixed_mode_ == 2) {
+      // max |P| of this density (both spins), and the routed count from 0
+      if (launch_zero_u32((uint32_t *)(d_mix_ + 1), 2, stream_)) fatal("route counter reset", 0);
+      for (int sp = 0; sp < kt.nspin; sp++)
+         if (launch_absmax(kt.P[sp], FP_size_, d_mix_, sp == 0, stream_)) fatal("max |P| launch", 0);
    }
    // debug bit 2: wait for each quartet type and report it on stderr
    const bool progress = (debug_ & 4) != 0;
@@ -1253,6 +1399,75 @@ void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
                    ++done, nty, types_[D.type].plan.L, D.n_groups, D.chunks.size(), sec);
       std::fflush(stderr);
    }
+}
+
+void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
+{
+   RunBufs w{};
+   KTargets kt{};
+   if (prepare_exec(devs, max_index, w, kt, FP_size_)) launch_all(devs, w, kt);
+}
+
+// ---------------------------------------------------------------------------
+// the resident plan's launches as one HIP graph, captured at its first
+// execution and replayed while the buffers and targets it was captured with
+// are unchanged
+// ---------------------------------------------------------------------------
+void Engine::drop_graph()
+{
+   if (graph_exec_) {
+      if (stream_) (void)hipStreamSynchronize((hipStream_t)stream_);
+      (void)hipGraphExecDestroy((hipGraphExec_t)graph_exec_);
+   }
+   graph_exec_ = nullptr;
+}
+
+void Engine::replay_graph(const RunBufs &w, const KTargets &kt)
+{
+   const uint64_t key[] = {(uint64_t)w.ac0, (uint64_t)w.fix, (uint64_t)w.fix_ws, (uint64_t)w.fixcnt,
+                           (uint64_t)w.gws.ptr, (uint64_t)w.gws.doubles, (uint64_t)w.fix_cap,
+                           (uint64_t)kt.P[0], (uint64_t)kt.P[1], (uint64_t)kt.K[0], (uint64_t)kt.K[1],
+                           (uint64_t)kt.nspin, graph_bits(kt.hf_fac), (uint64_t)kt.out, (uint64_t)d_data_,
+                           (uint64_t)d_C0_, (uint64_t)d_neighs_, (uint64_t)neighs_.size(), graph_bits(R_cut_),
+                           (uint64_t)truncated_, (uint64_t)debug_, (uint64_t)ld_C0_, (uint64_t)mixed_mode_,
+                           graph_bits(mixed_tau_), (uint64_t)d_mix_};
+   constexpr size_t NK = sizeof key / sizeof key[0];
+   static_assert(NK <= sizeof graph_key_ / sizeof graph_key_[0], "graph key");
+   bool same = graph_exec_ != nullptr;
+   for (size_t i = 0; same && i < NK; i++) same = graph_key_[i] == key[i];
+   if (!same) {
+      drop_graph();
+      const uint64_t s0 = n_slots_, p0 = n_prmc_;
+      const double f1 = timing_.flops[1], f2 = timing_.flops[2], f4 = timing_.flops[4];
+      hipGraph_t g = nullptr;
+      HIPCHECK(hipStreamBeginCapture((hipStream_t)stream_, hipStreamCaptureModeThreadLocal));
+      launch_all(plan_dev_, w, kt);
+      HIPCHECK(hipStreamEndCapture((hipStream_t)stream_, &g));
+      hipGraphExec_t ex = nullptr;
+      HIPCHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      HIPCHECK(hipGraphDestroy(g));
+      graph_exec_ = ex;
+      for (size_t i = 0; i < NK; i++) graph_key_[i] = key[i];
+      // host-side counts of one execution, added at every replay
+      graph_counts_[0] = n_slots_ - s0;
+      graph_counts_[1] = n_prmc_ - p0;
+      graph_flops_[0] = timing_.flops[1] - f1;
+      graph_flops_[1] = timing_.flops[2] - f2;
+      graph_flops_[2] = timing_.flops[4] - f4;
+      n_slots_ = s0;
+      n_prmc_ = p0;
+      timing_.flops[1] = f1;
+      timing_.flops[2] = f2;
+      timing_.flops[4] = f4;
+      graph_captures_++;
+   }
+   HIPCHECK(hipGraphLaunch((hipGraphExec_t)graph_exec_, (hipStream_t)stream_));
+   n_slots_ += graph_counts_[0];
+   n_prmc_ += graph_counts_[1];
+   timing_.flops[1] += graph_flops_[0];
+   timing_.flops[2] += graph_flops_[1];
+   timing_.flops[4] += graph_flops_[2];
+   graph_replays_++;
 }
 
 // The resident plan, like plan_execute, but into the caller's device P and
@@ -1290,7 +1505,7 @@ int Engine::plan_execute_noalloc(const double *Pa, const double *Pb, double *Ka,
    if (ncp_max > (size_t)MAX_NCP) return err(-4, "block size beyond the fix-up workspace", (long long)ncp_max);
    char *base = (char *)noalloc_scratch_;
    if (!base) return err(-8, "the no-allocation scratch array has no device address", 0);
-   RunBufs w;
+   RunBufs w{};
    w.ac0 = (double *)(base + NOALLOC_OFF_AC0);
    w.fix = (uint32_t *)(base + NOALLOC_OFF_FIX);
    w.fixcnt = (uint32_t *)base;
@@ -1298,7 +1513,7 @@ int Engine::plan_execute_noalloc(const double *Pa, const double *Pb, double *Ka,
    w.fix_cap = (uint32_t)NOALLOC_FIX;
    w.stats = nullptr;
    w.count = false;
-   KTargets kt;
+   KTargets kt{};
    kt.nspin = Pb ? 2 : 1;
    kt.hf_fac = hf_fac_;
    kt.out = d_out_;
@@ -1318,6 +1533,341 @@ int Engine::plan_execute_noalloc(const double *Pa, const double *Pb, double *Ka,
       for (auto &c : D.chunks_na) run_chunk(ti, D, c, w, kt);
    }
    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: per class of a quartet type, the schedule of its contraction,
+// HRR, spherical transform and K update, chosen by timing on a sample of the
+// plan (first execution after plan_end):
+//   0  the templated one-wavefront-per-set-quartet kernel (k_tqw), where
+//      the class has one,
+//   1  the workgroup kernel (k_gqs) with its own contraction loop,
+//   2  k_gqs after the matrix-core contraction (k_eco), where the class has
+//      a place in the contraction buffer.
+// Decisions are kept per (class, contractions, primitive quartets per row
+// in powers of two) for the life of the process and reused by later plans.
+// ---------------------------------------------------------------------------
+static std::mutex g_tune_mu;
+static std::map<uint64_t, int> g_tune;   // key -> schedule
+
+static uint64_t tune_key(int code, int nq, size_t n_prm, size_t n_rows)
+{
+   const double avg = n_rows ? (double)n_prm / (double)n_rows : 1.0;
+   const int bucket = std::min(15, std::max(0, (int)std::floor(std::log2(std::max(avg, 1.0)))));
+   return (uint64_t)code | (uint64_t)nq << 8 | (uint64_t)bucket << 20;
+}
+
+void Engine::apply_schedule(TypeInfo &ti, int k, int v)
+{
+   CqLayout &y = ti.lays[k];
+   y.skip = v == 0 ? 1 : 0;
+   y.eco = v == 2 ? 1 : 0;
+}
+
+void Engine::upload_layouts(TypeInfo &ti)
+{
+   assign_parts(ti.plan, ti.cplans, ti.lays);
+   make_eco_items(ti);
+   HIPCHECK(hipMemcpy(ti.d_lays, ti.lays.data(), sizeof(CqLayout) * ti.lays.size(), hipMemcpyHostToDevice));
+   if (ti.d_eco_items) HIPCHECK(hipFree(ti.d_eco_items));
+   ti.d_eco_items = nullptr;
+   if (!ti.eco_items.empty()) {
+      HIPCHECK(hipMalloc((void **)&ti.d_eco_items, sizeof(EcoItem) * ti.eco_items.size()));
+      HIPCHECK(hipMemcpy(ti.d_eco_items, ti.eco_items.data(), sizeof(EcoItem) * ti.eco_items.size(),
+                         hipMemcpyHostToDevice));
+   }
+}
+
+void Engine::autotune(const RunBufs &w, const KTargets &kt)
+{
+   synchronize();
+   const auto t0 = std::chrono::steady_clock::now();
+   // a scratch K: the timed launches must not touch the caller's
+   double *Ks = nullptr;
+   HIPCHECK(hipMalloc((void **)&Ks, sizeof(double) * std::max<size_t>(FP_size_, 1) * (size_t)kt.nspin));
+   KTargets ks = kt;
+   ks.K[0] = Ks;
+   ks.K[1] = kt.nspin > 1 ? Ks + FP_size_ : nullptr;
+   ks.out = nullptr;
+   hipStream_t st = (hipStream_t)stream_;
+   hipEvent_t e0, e1;
+   HIPCHECK(hipEventCreate(&e0));
+   HIPCHECK(hipEventCreate(&e1));
+   EcoItem *d_items = nullptr;
+   size_t items_cap = 0;
+   const DevTables t = dev_tables();
+   for (auto &D : plan_dev_) {
+      if (D.chunks.empty()) continue;
+      TypeInfo &ti = types_[D.type];
+      const TypePlan &tp = ti.plan;
+      // decisions already known for every class of the type
+      std::vector<int> dec(tp.nclass, -1);
+      bool all = true;
+      {
+         std::lock_guard<std::mutex> lk(g_tune_mu);
+         for (int k = 0; k < tp.nclass; k++) {
+            auto it = g_tune.find(tune_key(tp.cls_code[k], tp.cls_nl[k], D.n_prm, D.n_rows));
+            if (it != g_tune.end()) dec[k] = it->second;
+            else if (D.class_used[k]) all = false;
+         }
+      }
+      if (!all) {
+         // the sample: the first set quartets of the first chunk, about
+         // TUNE_PRM primitive quartets (launches long enough to time), at
+         // most TUNE_GROUPS set quartets
+         const Chunk &c0 = D.chunks[0];
+         Chunk c = c0;
+         const double ppg = (double)(c0.p1 - c0.p0) / (double)std::max<uint32_t>(c0.g1 - c0.g0, 1);
+         const uint32_t ng = (uint32_t)std::max(1.0, std::min<double>(TUNE_GROUPS, TUNE_PRM / std::max(ppg, 1.0)));
+         c.g1 = std::min<uint32_t>(c0.g1, c0.g0 + ng);
+         if (c.g1 < c0.g1) {
+            GroupRec gl;
+            HIPCHECK(hipMemcpy(&gl, D.groups + (c.g1 - 1), sizeof gl, hipMemcpyDeviceToHost));
+            RowRec rl;
+            HIPCHECK(hipMemcpy(&rl, D.rows + (gl.row1 - 1), sizeof rl, hipMemcpyDeviceToHost));
+            c.r1 = gl.row1;
+            c.p1 = rl.prm_off + rl.n_prm;
+         }
+         BatchDev b{};   // every field zero / null unless set below
+         b.tp = ti.d_plan;
+         b.e1 = ti.d_e1;
+         b.e2 = ti.d_e2;
+         b.rows = D.rows;
+         b.groups = D.groups;
+         b.prm_row = D.prm_row;
+         b.prm_ip = D.prm_ip;
+         b.ksg = D.ksg;
+         b.kss = D.kss;
+         b.ksx = D.ksx;
+         b.data = d_data_;
+         b.ac0 = w.ac0;
+         b.p0 = c.p0;
+         b.p1 = c.p1;
+         b.r0 = c.r0;
+         b.r1 = c.r1;
+         b.g0 = c.g0;
+         b.g1 = c.g1;
+         b.stats = nullptr;
+         b.fix = w.fix;
+         b.fix_count = w.fixcnt;
+         b.fix_cap = w.fix_cap;
+         b.force_fix = 0;
+         b.fix_ws = w.fix_ws;
+         b.eco = w.ac0 + (size_t)(c.p1 - c.p0) * tp.ncp;
+         b.eco_row = ti.eco_row;
+         b.gmag = nullptr;    // no calibration and no single-precision route while timing
+         b.gflag = nullptr;
+         if (launch_zero_u32(w.fixcnt, 1, st) || launch_fm_r(tp, t, b, st) ||
+             launch_fixup(tp, t, b, std::min<uint32_t>(64u, w.fix_cap), st))
+            fatal("autotune: sample launch", 0);
+         const std::vector<CqLayout> keep = ti.lays;
+         for (int k = 0; k < tp.nclass; k++) {
+            if (!D.class_used[k] || dec[k] >= 0) continue;
+            double best = 1e300;
+            int bv = keep[k].skip ? 0 : (keep[k].eco ? 2 : 1);
+            // the sample's contraction FLOPs of the class: above TUNE_VALU_FLOPS
+            // the VALU contraction loop (schedule 1) is not timed where the
+            // matrix cores can take the class (it is slower there by an order
+            // of magnitude or two, and timing it would cost seconds)
+            const double cflops = 2.0 * tp.cls_nl[k] * (double)tp.cls_ncp[k] * (double)(c.p1 - c.p0);
+            for (int v = 0; v < 3; v++) {
+               if (v == 0 && !tq_has(tp.cls_code[k])) continue;
+               if (v == 2 && keep[k].eco_off < 0) continue;
+               if (v == 1 && keep[k].eco_off >= 0 && cflops > TUNE_VALU_FLOPS) continue;
+               // this class alone in part 0 with schedule v
+               std::vector<CqLayout> L = keep;
+               for (auto &y : L) y.part = -1;
+               L[k].skip = v == 0 ? 1 : 0;
+               L[k].eco = v == 2 ? 1 : 0;
+               L[k].part = 0;
+               HIPCHECK(hipMemcpy(ti.d_lays, L.data(), sizeof(CqLayout) * L.size(), hipMemcpyHostToDevice));
+               std::vector<EcoItem> items;
+               if (v == 2) {
+                  for (int jt0 = 0; jt0 * 16 < tp.cls_ncp[k]; jt0 += 4)
+                     for (int qb = 0; qb * 64 < tp.cls_nl[k]; qb++)
+                        items.push_back(EcoItem{(uint16_t)k, (uint16_t)jt0, (uint16_t)qb, 0});
+                  if (items.size() > items_cap) {
+                     if (d_items) HIPCHECK(hipFree(d_items));
+                     items_cap = items.size() * 2;
+                     HIPCHECK(hipMalloc((void **)&d_items, sizeof(EcoItem) * items_cap));
+                  }
+                  HIPCHECK(hipMemcpy(d_items, items.data(), sizeof(EcoItem) * items.size(), hipMemcpyHostToDevice));
+               }
+               // the faster of two runs: the first may load the kernel's code object
+               float ms = 1e30f;
+               for (int rep = 0; rep < 2; rep++) {
+                  float msr = 0.f;
+                  HIPCHECK(hipEventRecord(e0, st));
+                  int rc = 0;
+                  if (v == 0) {
+                     rc = launch_tq(k, tp, t, b, ks, ti.d_srcmaps + keep[k].o_src, st);
+                  } else {
+                     if (v == 2) {
+                        EcoArgs e{};
+                        e.items = d_items;
+                        e.n_items = (int)items.size();
+                        e.srcmaps = ti.d_srcmaps;
+                        e.src_off = ti.d_eco_info;
+                        e.eco_off = ti.d_eco_info + MAXTC;
+                        e.row_doubles = ti.eco_row;
+                        rc = launch_eco(tp, b, e, st);
+                     }
+                     if (!rc)
+                        rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, 0, 1, t, b, ks,
+                                        w.gws, st);
+                  }
+                  if (rc) fatal("autotune: candidate launch", rc);
+                  HIPCHECK(hipEventRecord(e1, st));
+                  HIPCHECK(hipEventSynchronize(e1));
+                  HIPCHECK(hipEventElapsedTime(&msr, e0, e1));
+                  ms = std::min(ms, msr);
+               }
+               if (ms < best) {
+                  best = ms;
+                  bv = v;
+               }
+            }
+            dec[k] = bv;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[tune_key(tp.cls_code[k], tp.cls_nl[k], D.n_prm, D.n_rows)] = bv;
+         }
+      }
+      for (int k = 0; k < tp.nclass; k++)
+         if (dec[k] >= 0) {
+            apply_schedule(ti, k, dec[k]);
+            tune_counts_[dec[k]]++;
+         }
+      upload_layouts(ti);
+      if (debug_ & 4) {
+         std::fprintf(stderr, "libgint: autotune: quartet type L %d (%zu set quartets) %s at %.1f s\n", tp.L, D.n_groups,
+                      all ? "from earlier decisions" : "timed",
+                      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+         std::fflush(stderr);
+      }
+   }
+   HIPCHECK(hipStreamSynchronize(st));
+   if (d_items) HIPCHECK(hipFree(d_items));
+   HIPCHECK(hipFree(Ks));
+   HIPCHECK(hipEventDestroy(e0));
+   HIPCHECK(hipEventDestroy(e1));
+   tune_seconds_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+   plan_tuned_ = true;
+}
+
+// ---------------------------------------------------------------------------
+// serving (server.cpp): a client handle with a stream of its own, whose
+// resident plan can leave the device (host copy) and come back, executed
+// into device P / K arrays that the server keeps per client
+// ---------------------------------------------------------------------------
+void Engine::use_private_stream()
+{
+   if (own_stream_) return;
+   synchronize();
+   hipStream_t q = nullptr;
+   HIPCHECK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+   stream_ = q;
+   own_stream_ = true;
+   ensure_side();
+}
+
+uint64_t Engine::device_bytes() const
+{
+   uint64_t b = 0;
+   for (const auto &d : plan_dev_)
+      b += d.cap[0] * sizeof(RowRec) + d.cap[1] * sizeof(GroupRec) + d.cap[2] * 4 + d.cap[3] * 4 +
+           d.cap[4] * sizeof(KsGroup) + d.cap[5] * sizeof(KsRec) + d.cap[6] * sizeof(KsSet);
+   b += work_bytes_ + fix_cap_ * 8 + fix_ws_cap_ * 8 + scratch_cap_ * 8 + d_data_cap_ * 8 + C0_.size() * 8 +
+        d_neighs_cap_ * 8;
+   return b;
+}
+
+uint64_t Engine::work_estimate() const
+{
+   size_t ncp_max = 0, mcb = 0, scr = 0;
+   for (const auto &D : plan_dev_) {
+      if (D.chunks.empty()) continue;
+      const TypeInfo &ti = types_[D.type];
+      ncp_max = std::max(ncp_max, (size_t)ti.plan.ncp);
+      mcb = std::max(mcb, D.max_chunk_bytes);
+      scr = std::max(scr, (size_t)ti.cq_gws * SCRATCH_GRID * GQS_PARTS);
+   }
+   if (!ncp_max) return 0;
+   return std::max<size_t>(mcb, 1 << 20) + 2 * 4 * 4096 + 4096 * (ncp_max + 16) * 8 + scr * 8;
+}
+
+uint64_t Engine::plan_host_bytes() const
+{
+   uint64_t b = 0;
+   for (const auto &Q : plan_host_)
+      b += Q.rows.size() * sizeof(RowRec) + Q.groups.size() * sizeof(GroupRec) + Q.prm_ip.size() * 8 +
+           Q.ksg.size() * sizeof(KsGroup) + Q.ksx.size() * sizeof(KsRec) + Q.kss.size() * sizeof(KsSet);
+   return b;
+}
+
+template <class T>
+static void dev_get(std::vector<T> &v, const T *ptr, size_t n)
+{
+   v.resize(n);
+   if (n) HIPCHECK(hipMemcpy(v.data(), ptr, sizeof(T) * n, hipMemcpyDeviceToHost));
+}
+
+void Engine::plan_evict()
+{
+   if (recording_) fatal("plan_evict while recording", 0);
+   synchronize();
+   drop_graph();
+   plan_host_.assign(plan_dev_.size(), TypeQueue());
+   for (size_t ty = 0; ty < plan_dev_.size(); ty++) {
+      TypeDev &D = plan_dev_[ty];
+      if (D.n_groups) {
+         TypeQueue &Q = plan_host_[ty];
+         dev_get(Q.rows, D.rows, D.n_rows);
+         dev_get(Q.groups, D.groups, D.n_groups);
+         dev_get(Q.prm_ip, D.prm_ip, D.n_prm);
+         dev_get(Q.prm_row, D.prm_row, D.n_prm);
+         dev_get(Q.ksg, D.ksg, D.n_ksg);
+         dev_get(Q.ksx, D.ksx, D.n_ksx);
+         dev_get(Q.kss, D.kss, D.n_kss);
+      }
+      D.release();
+   }
+   plan_dev_.clear();
+   mixed_calibrated_ = false;
+   void *bufs[] = {d_work_, d_fix_, d_fix_ws_, d_scratch_};
+   for (void *b : bufs)
+      if (b) HIPCHECK(hipFree(b));
+   d_work_ = nullptr;
+   d_fix_ = nullptr;
+   d_fix_ws_ = nullptr;
+   d_scratch_ = nullptr;
+   work_bytes_ = fix_cap_ = fix_ws_cap_ = scratch_cap_ = 0;
+   evicted_ = true;
+}
+
+void Engine::plan_restore()
+{
+   if (!evicted_) return;
+   upload(plan_host_, plan_dev_, true);
+   plan_host_.clear();
+   evicted_ = false;
+}
+
+void Engine::plan_execute_into(const double *P0, const double *P1, double *K0, double *K1, int nspin, uint64_t n)
+{
+   if (recording_) fatal("plan_execute_into while recording", 0);
+   if (evicted_) fatal("plan_execute_into with the plan off the device", 0);
+   RunBufs w{};
+   KTargets kt{};
+   if (!prepare_exec(plan_dev_, plan_max_index_, w, kt, n)) return;
+   kt.nspin = nspin;
+   kt.P[0] = P0;
+   kt.P[1] = P1;
+   kt.K[0] = K0;
+   kt.K[1] = K1;
+   if (graph_on_ && nomp_ == 1 && !timing_on_ && !stats_on_ && (debug_ & 6) == 0)
+      replay_graph(w, kt);
+   else
+      launch_all(plan_dev_, w, kt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1341,7 +1891,7 @@ Timing Engine::timing()
 void Engine::timing_detail(int cat, double out[256])
 {
    synchronize();
-   for (int i = 0; i < 256; i++) out[i] = (cat >= 0 && cat < 4) ? detail_[cat][i] : 0.0;
+   for (int i = 0; i < 256; i++) out[i] = (cat >= 0 && cat < 5) ? detail_[cat][i] : 0.0;
 }
 
 void Engine::stats(uint64_t out[8])

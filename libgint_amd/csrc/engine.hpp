@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include <array>
+#include <cstring>
 #include <map>
 #include <unordered_map>
 #include <vector>
@@ -60,7 +61,7 @@ struct TypeQueue {
 
 struct Chunk {
    uint32_t g0, g1, r0, r1, p0, p1;
-   double tq_flops, k3_flops;
+   double tq_flops, k3_flops, eco_flops;
 };
 
 // one type's work on the device
@@ -75,10 +76,13 @@ struct TypeDev {
    size_t cap[7] = {0, 0, 0, 0, 0, 0, 0};   // allocated elements of each array
    size_t n_rows = 0, n_groups = 0, n_prm = 0, n_ksx = 0, n_ksg = 0, n_kss = 0;
    size_t max_group_bytes = 0;            // largest set quartet's primitive blocks
+   size_t max_chunk_bytes = 0;            // largest chunk's blocks and contraction buffer rows
    std::vector<Chunk> chunks;
    std::vector<Chunk> chunks_na;          // the same work in chunks that fit the no-allocation scratch
    bool na_fit = true;                    // every set quartet fits it
    std::vector<char> class_used;
+   double *gmag = nullptr;      // mixed precision: calibrated max |block element| per set quartet
+   uint32_t *gflag = nullptr;   // mixed precision: route per set quartet (1 single precision)
    void release();
 };
 
@@ -95,6 +99,13 @@ struct TypeInfo {
    ClassPlan *d_cps = nullptr;
    CqLayout *d_lays = nullptr;
    int *d_srcmaps = nullptr;
+   // matrix-core contraction (k_eco): work items of a row, [class map
+   // offsets, class buffer offsets], doubles per row of the buffer
+   std::vector<EcoItem> eco_items;
+   std::vector<int> eco_info;
+   size_t eco_row = 0;
+   EcoItem *d_eco_items = nullptr;
+   int *d_eco_info = nullptr;
 };
 
 // device buffers of one chunk's kernels
@@ -109,9 +120,9 @@ struct RunBufs {
 };
 
 struct Timing {
-   double ms[4] = {0, 0, 0, 0};        // fm_r, tq, gqs, fixup
-   uint64_t launches[4] = {0, 0, 0, 0};
-   double flops[4] = {0, 0, 0, 0};     // host-counted algorithmic FLOPs of tq [1] and gqs [2]
+   double ms[5] = {0, 0, 0, 0, 0};        // fm_r, tq, gqs, fixup, eco (matrix-core contraction)
+   uint64_t launches[5] = {0, 0, 0, 0, 0};
+   double flops[5] = {0, 0, 0, 0, 0};     // host-counted algorithmic FLOPs of tq [1], gqs [2] and eco [4]
 };
 
 class Engine {
@@ -157,6 +168,23 @@ class Engine {
    void timing_detail(int cat, double out[256]);
    void reset_counters();
    void set_debug(int flags) { debug_ = flags; }
+   void set_graph(bool on) { graph_on_ = on; }
+   void set_autotune(bool on) { autotune_on_ = on; }
+   void set_mixed_precision(double tau) { mixed_tau_ = tau > 0.0 ? tau : 0.0; }
+   void mixed_info(double out[4]);
+   void tune_info(double out[5]) const
+   {
+      out[0] = plan_tuned_ ? 1.0 : 0.0;
+      for (int i = 0; i < 3; i++) out[1 + i] = (double)tune_counts_[i];
+      out[4] = tune_seconds_;
+   }
+   void graph_info(uint64_t out[4]) const
+   {
+      out[0] = graph_on_ ? 1 : 0;
+      out[1] = graph_captures_;
+      out[2] = graph_replays_;
+      out[3] = graph_exec_ ? 1 : 0;
+   }
    void plan_begin();
    void plan_end();
    void plan_execute();
@@ -165,6 +193,17 @@ class Engine {
    int plan_execute_noalloc(const double *Pa, const double *Pb, double *Ka, double *Kb, uint64_t n);
    void plan_clear();
    void plan_info(uint64_t out[6]);
+   // serving (server.cpp)
+   void use_private_stream();
+   void *stream() const { return stream_; }
+   uint64_t device_bytes() const;       // device memory held now (plan, work buffers, tables, basis)
+   uint64_t plan_host_bytes() const;    // the evicted plan's host copy
+   uint64_t work_estimate() const;      // work buffers an execution of the resident plan allocates
+   uint64_t plan_max_index() const { return plan_max_index_; }
+   bool plan_evicted() const { return evicted_; }
+   void plan_evict();                   // plan records and work buffers off the device, host copy kept
+   void plan_restore();
+   void plan_execute_into(const double *P0, const double *P1, double *K0, double *K1, int nspin, uint64_t n);
    void queue_info(uint64_t out[8]);
    int queue_types(uint64_t *out, int max_types);
    int np_of(int i) const { return i < (int)sets_.size() ? sets_[i].np : 0; }
@@ -177,6 +216,19 @@ class Engine {
    bool set_level(const KsGroup *g, const TypePlan &tp, KsSet &s) const;
    void upload(std::vector<TypeQueue> &qs, std::vector<TypeDev> &devs, bool release_host);
    void execute(std::vector<TypeDev> &devs, uint64_t max_index);
+   bool prepare_exec(std::vector<TypeDev> &devs, uint64_t max_index, RunBufs &w, KTargets &kt, uint64_t n);
+   void launch_all(std::vector<TypeDev> &devs, const RunBufs &w, const KTargets &kt);
+   void replay_graph(const RunBufs &w, const KTargets &kt);
+   void autotune(const RunBufs &w, const KTargets &kt);
+   void apply_schedule(TypeInfo &ti, int k, int v);
+   void upload_layouts(TypeInfo &ti);
+   void drop_graph();
+   static uint64_t graph_bits(double x)
+   {
+      uint64_t u;
+      std::memcpy(&u, &x, 8);
+      return u;
+   }
    void run_chunk(const TypeInfo &ti, const TypeDev &td, const Chunk &c, const RunBufs &w, const KTargets &kt);
    DevTables dev_tables() const;
    void ensure_work(size_t ncp_max, size_t max_prm_chunk, size_t max_group_bytes);
@@ -264,6 +316,35 @@ class Engine {
    double *d_P_[2] = {nullptr, nullptr};
    double *d_K_[2] = {nullptr, nullptr};
 
+   // autotuner: the schedule of each class, chosen at the plan's first execution
+   static constexpr uint32_t TUNE_GROUPS = 16384;  // at most this many set quartets in a type's timing sample
+   static constexpr double TUNE_PRM = 262144;       // primitive quartets aimed at in the sample
+   static constexpr double TUNE_VALU_FLOPS = 2e8;   // largest sample contraction timed on the VALU
+   bool autotune_on_ = true;
+   bool plan_tuned_ = false;
+   uint64_t tune_counts_[3] = {0, 0, 0};          // classes given k_tqw / k_gqs / k_eco + k_gqs
+   double tune_seconds_ = 0.0;
+
+   // mixed precision (plans): set quartets whose calibrated bound times
+   // max |P| is below tau take the single-precision contraction
+   double mixed_tau_ = 0.0;
+   bool mixed_calibrated_ = false;
+   int mixed_mode_ = 0;                    // of the execution being launched: 0 off, 1 calibrate, 2 route
+   unsigned long long *d_mix_ = nullptr;   // [0] max |P| bits, [1] set quartets routed to single precision
+   void ensure_mixed();
+
+   bool own_stream_ = false;
+   bool evicted_ = false;
+   std::vector<TypeQueue> plan_host_;
+
+   // the resident plan as a HIP graph (plan_execute)
+   bool graph_on_ = true;
+   void *graph_exec_ = nullptr;
+   uint64_t graph_key_[32] = {};
+   uint64_t graph_counts_[2] = {0, 0};
+   double graph_flops_[3] = {0, 0, 0};
+   uint64_t graph_captures_ = 0, graph_replays_ = 0;
+
    bool timing_on_ = false;
    Timing timing_;
    struct EvRec {
@@ -272,7 +353,7 @@ class Engine {
       int key;
    };
    int ev_key_ = 0;
-   std::array<std::array<double, 256>, 4> detail_{};   // ms per category and key (type L / class code)
+   std::array<std::array<double, 256>, 5> detail_{};   // ms per category and key (type L / class code)
    std::vector<void *> ev_pool_;
    std::vector<EvRec> ev_live_;
    size_t ev_used_ = 0;

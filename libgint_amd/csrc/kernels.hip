@@ -162,50 +162,50 @@ __device__ void prim_prepare(const double *Ao, const double *Bo, const double *C
    const double inv_zcd = 1. / zcd;
    const double z = zab + zcd;
    const double inv_z = 1. / z;
-   double A[3], B[3], C[3], D[3], ABs[3], CDs[3], PQs[3], shift[3], Cs[3], Ds[3], P[3], Q[3];
-   gint_pbc(Ao, Bo, t.cell, ABs);
+   double A[3], B[3], C[3], D[3], bimg[3], dimg[3], qimg[3], kshift[3], Cn[3], Dn[3], P[3], Q[3];
+   gint_pbc(Ao, Bo, t.cell, bimg);
    A[0] = Ao[0]; A[1] = Ao[1]; A[2] = Ao[2];
-   B[0] = Ao[0] + ABs[0] + neighs[n1 * 3 + 0];
-   B[1] = Ao[1] + ABs[1] + neighs[n1 * 3 + 1];
-   B[2] = Ao[2] + ABs[2] + neighs[n1 * 3 + 2];
+   B[0] = Ao[0] + bimg[0] + neighs[n1 * 3 + 0];
+   B[1] = Ao[1] + bimg[1] + neighs[n1 * 3 + 1];
+   B[2] = Ao[2] + bimg[2] + neighs[n1 * 3 + 2];
    P[0] = (za * A[0] + zb * B[0]) * inv_zab;
    P[1] = (za * A[1] + zb * B[1]) * inv_zab;
    P[2] = (za * A[2] + zb * B[2]) * inv_zab;
-   gint_pbc(Co, Do, t.cell, CDs);
+   gint_pbc(Co, Do, t.cell, dimg);
    C[0] = Co[0]; C[1] = Co[1]; C[2] = Co[2];
-   D[0] = Co[0] + CDs[0] + neighs[n2 * 3 + 0];
-   D[1] = Co[1] + CDs[1] + neighs[n2 * 3 + 1];
-   D[2] = Co[2] + CDs[2] + neighs[n2 * 3 + 2];
+   D[0] = Co[0] + dimg[0] + neighs[n2 * 3 + 0];
+   D[1] = Co[1] + dimg[1] + neighs[n2 * 3 + 1];
+   D[2] = Co[2] + dimg[2] + neighs[n2 * 3 + 2];
    Q[0] = (zc * C[0] + zd * D[0]) * inv_zcd;
    Q[1] = (zc * C[1] + zd * D[1]) * inv_zcd;
    Q[2] = (zc * C[2] + zd * D[2]) * inv_zcd;
-   gint_pbc(Q, P, t.cell, PQs);
-   shift[0] = P[0] - Q[0] - PQs[0];
-   shift[1] = P[1] - Q[1] - PQs[1];
-   shift[2] = P[2] - Q[2] - PQs[2];
-   const double normf = GINT_TWO_PI_52 * inv_zab * inv_zcd;
+   gint_pbc(Q, P, t.cell, qimg);
+   kshift[0] = P[0] - Q[0] - qimg[0];
+   kshift[1] = P[1] - Q[1] - qimg[1];
+   kshift[2] = P[2] - Q[2] - qimg[2];
+   const double pref = GINT_TWO_PI_52 * inv_zab * inv_zcd;
    double dx = A[0] - B[0], dy = A[1] - B[1], dz = A[2] - B[2];
-   const double RAB2 = dx * dx + dy * dy + dz * dz;
-   const double expon_ab = -(za * zb) / (za + zb) * RAB2;
+   const double ab2 = dx * dx + dy * dy + dz * dz;
+   const double eab = -(za * zb) / (za + zb) * ab2;
    dx = C[0] - D[0]; dy = C[1] - D[1]; dz = C[2] - D[2];
-   const double RCD2 = dx * dx + dy * dy + dz * dz;
-   const double expon_cd = -(zc * zd) / (zc + zd) * RCD2;
-   const double Zn = 1. / sqrt(z);
-   g.Kfac = Zn * normf * gint_exp(expon_ab + expon_cd);
-   Cs[0] = Co[0] + shift[0];
-   Cs[1] = Co[1] + shift[1];
-   Cs[2] = Co[2] + shift[2];
-   Ds[0] = Co[0] + CDs[0] + neighs[n2 * 3 + 0] + shift[0];
-   Ds[1] = Co[1] + CDs[1] + neighs[n2 * 3 + 1] + shift[1];
-   Ds[2] = Co[2] + CDs[2] + neighs[n2 * 3 + 2] + shift[2];
-   Q[0] = (zc * Cs[0] + zd * Ds[0]) * inv_zcd;
-   Q[1] = (zc * Cs[1] + zd * Ds[1]) * inv_zcd;
-   Q[2] = (zc * Cs[2] + zd * Ds[2]) * inv_zcd;
+   const double cd2 = dx * dx + dy * dy + dz * dz;
+   const double ecd = -(zc * zd) / (zc + zd) * cd2;
+   const double rsz = 1. / sqrt(z);
+   g.Kfac = rsz * pref * gint_exp(eab + ecd);
+   Cn[0] = Co[0] + kshift[0];
+   Cn[1] = Co[1] + kshift[1];
+   Cn[2] = Co[2] + kshift[2];
+   Dn[0] = Co[0] + dimg[0] + neighs[n2 * 3 + 0] + kshift[0];
+   Dn[1] = Co[1] + dimg[1] + neighs[n2 * 3 + 1] + kshift[1];
+   Dn[2] = Co[2] + dimg[2] + neighs[n2 * 3 + 2] + kshift[2];
+   Q[0] = (zc * Cn[0] + zd * Dn[0]) * inv_zcd;
+   Q[1] = (zc * Cn[1] + zd * Dn[1]) * inv_zcd;
+   Q[2] = (zc * Cn[2] + zd * Dn[2]) * inv_zcd;
    for (int k = 0; k < 3; k++) {
       g.P[k] = P[k];
       g.Q[k] = Q[k];
       g.PA[k] = P[k] - A[k];
-      g.QC[k] = Q[k] - Cs[k];
+      g.QC[k] = Q[k] - Cn[k];
    }
    g.zab = zab;
    g.zcd = zcd;
@@ -638,10 +638,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
          if (q >= nprim) break;
          const double *Gq = lds + GEO * q;
          double *dst = FIX ? b.fix_ws + (size_t)w * (tp.ncp + 16) : b.ac0 + (size_t)(w0 + q) * tp.ncp;
+         // calibration of the mixed-precision routing: the largest block
+         // element of each set quartet (a positive double orders as its bits)
+         unsigned long long *gm = nullptr;
+         if (!FIX && b.gmag) gm = (unsigned long long *)(b.gmag + b.rows[b.prm_row[b.p0 + w0 + q]].group);
          if (!FIX && !any_surv[q]) {
             // no cell survives: the blocks are zero but for the screened
             // slots' Fm[0] in (00|00)
             for (int j = lane; j < tp.ncp; j += 64) dst[j] = (j == 0 && tp.has00) ? qsum[q] : 0.0;
+            if (gm && lane == 0 && tp.has00) atomicMax(gm, (unsigned long long)__double_as_longlong(fabs(qsum[q])));
             __syncthreads();
             continue;
          }
@@ -697,6 +702,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
          }
          __syncthreads();
          // (e0|f0) = sum E^e_t E^e_u E^e_v W[tuv][f]
+         double vmax = 0.0;
          for (int j = lane; j < tp.ncp; j += 64) {
             const uint32_t it = e2[j];
             const int ex = it & 7, ey = (it >> 3) & 7, ez = (it >> 6) & 7, col = (int)((it >> 9) & 127);
@@ -714,6 +720,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const int jd = (int)(it >> 16);
             if (!FIX && jd == 0 && tp.has00) s += qsum[q];
             dst[jd] = s;
+            vmax = fmax(vmax, fabs(s));
+         }
+         if (gm) {
+            for (int off = 32; off > 0; off >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, off, 64));
+            if (lane == 0) atomicMax(gm, (unsigned long long)__double_as_longlong(vmax));
          }
          __syncthreads();
       }
@@ -758,6 +769,8 @@ constexpr int CQ_PCH = 8;    // primitive quartets per contraction step
 // [7][28][3] tables
 constexpr int CQ_TAB_C2S = 0, CQ_TAB_FIRST = 8 * GINT_C2S_SIZE, CQ_TAB_MINUS = CQ_TAB_FIRST + 40,
               CQ_TAB_PLUS = CQ_TAB_MINUS + 120, CQ_TAB_BYTES = (CQ_TAB_PLUS + 588 + 15) & ~15;
+// the classes of a k_gqs part, listed in LDS after the tables
+constexpr int CQ_PLIST_BYTES = (4 * (MAXTC + 1) + 15) & ~15;
 
 
 __device__ __forceinline__ double *lds_base()
@@ -850,13 +863,22 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
       for (uint32_t r = g.row0; r < g.row1; r++) {
          const RowRec row = b.rows[r];
          // ---- contraction of the row's primitive quartets into buf0 ----
-         for (int idx = tid; idx < nb * ncp; idx += NT) {
-            const int l = fdiv(idx, ncp);
-            WS[l * lws + (idx - l * ncp)] = 0.0;
+         if (y.eco) {
+            // done on the matrix cores (k_eco): the row's contracted blocks
+            const double *E = b.eco + (size_t)(r - b.r0) * b.eco_row + y.eco_off + (size_t)l0 * ncp;
+            for (int idx = tid; idx < nb * ncp; idx += NT) {
+               const int l = fdiv(idx, ncp);
+               WS[l * lws + (idx - l * ncp)] = E[idx];
+            }
          }
          const uint32_t *IP = b.prm_ip + row.prm_off;
          const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * ncpt;
-         for (int pc = 0; pc < (int)row.n_prm; pc += CQ_PCH) {
+         if (!y.eco)
+            for (int idx = tid; idx < nb * ncp; idx += NT) {
+               const int l = fdiv(idx, ncp);
+               WS[l * lws + (idx - l * ncp)] = 0.0;
+            }
+         for (int pc = 0; pc < (y.eco ? 0 : (int)row.n_prm); pc += CQ_PCH) {
             const int np = min(CQ_PCH, (int)row.n_prm - pc);
             __syncthreads();
             if (tid < np) ipb[tid] = IP[pc + tid];
@@ -1131,10 +1153,13 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
    }
 }
 
-// one workgroup per set quartet, its classes in turn: the group's rows,
-// primitive lists and coefficients stay hot in the caches across classes
-// at most 128 VGPRs (4 wavefronts per SIMD): the kernel waits on memory, and
-// more wavefronts in flight outweigh the few values it then keeps in scratch
+// Work items are (set quartet, class of this part) pairs, the classes of a
+// set quartet on consecutive items: a type with few, large set quartets (an
+// spdf centre's 16^4 primitive quartets and 256 classes) still fills the
+// chip, and the set quartet's blocks are read by neighbouring workgroups
+// while they sit in L2.  at most 128 VGPRs (4 wavefronts per SIMD): the
+// kernel waits on memory, and more wavefronts in flight outweigh the few
+// values it then keeps in scratch
 template <int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_gqs(const ClassPlan *__restrict__ cps, const CqLayout *__restrict__ lays,
                                                     const int *__restrict__ srcmaps, const TypePlan *__restrict__ tpp,
@@ -1143,46 +1168,57 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 {
    const TypePlan &tp = *tpp;
    const int tid = threadIdx.x;
-   // LDS: [class workspace, cls_bytes][class plan][coefficients nco_tot][primitive indices CQ_PCH][tables]
+   // LDS: [class workspace, cls_bytes][class plan][coefficients nco_tot][primitive indices CQ_PCH][tables][classes of the part]
    ClassPlan *cpl = (ClassPlan *)((char *)lds_base() + cls_bytes);
    double *co = (double *)((char *)cpl + CPL_BYTES);
    uint32_t *ipb = (uint32_t *)(co + tp.nco_tot);
    char *tabs = (char *)lds_base() + ((cls_bytes + CPL_BYTES + 8 * tp.nco_tot + 4 * CQ_PCH + 15) & ~15);
+   int *plist = (int *)(tabs + CQ_TAB_BYTES);   // [MAXTC + 1]: count, then the part's classes
    for (int i = tid; i < GINT_C2S_SIZE; i += NT) ((double *)(tabs + CQ_TAB_C2S))[i] = c_c2s[i];
    for (int i = tid; i < 40; i += NT) ((unsigned char *)(tabs + CQ_TAB_FIRST))[i] = c_cart.first[i / 10][i % 10];
    for (int i = tid; i < 120; i += NT)
       ((signed char *)(tabs + CQ_TAB_MINUS))[i] = c_cart.minus[i / 30][(i / 3) % 10][i % 3];
    for (int i = tid; i < 588; i += NT) ((signed char *)(tabs + CQ_TAB_PLUS))[i] = c_cart.plus[i / 84][(i / 3) % 28][i % 3];
+   if (tid == 0) {
+      int n = 0;
+      for (int k = 0; k < tp.nclass; k++)
+         if (!lays[k].skip && lays[k].part == part) plist[1 + n++] = k;
+      plist[0] = n;
+   }
    __syncthreads();
-   for (uint32_t gi = b.g0 + blockIdx.x; gi < b.g1; gi += gridDim.x) {
+   const uint32_t npc = (uint32_t)plist[0];
+   const uint32_t nitems = (b.g1 - b.g0) * npc;
+   uint32_t cur_g = 0xffffffffu;
+   for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
+      const uint32_t gi = b.g0 + item / npc;
+      const int k = plist[1 + item % npc];
       const GroupRec &g = b.groups[gi];
       const bool setlevel = (g.ksmode & KS_SETLEVEL) != 0;
-      // the set quartet's contraction coefficients, shell by shell
-      for (int tc = 0; tc < 4; tc++) {
-         const int npt = ((g.np4 >> (4 * tc)) & 15) + 1;
-         for (int s = 0; s < tp.nsh[tc]; s++) {
-            const int n = tp.nl[tc][s] * MAXNP;
-            for (int i = tid; i < n; i += NT) {
-               const int inl = i / MAXNP, ip = i % MAXNP;
-               co[tp.co_off[tc][s] + i] = ip < npt ? b.data[g.Kof[tc][s] + inl * npt + ip] : 0.0;
+      if (!setlevel && ks_count(b.ksg[(size_t)g.ks + k].flags) == 0) continue;
+      __syncthreads();
+      if (gi != cur_g) {
+         // the set quartet's contraction coefficients, shell by shell
+         for (int tc = 0; tc < 4; tc++) {
+            const int npt = ((g.np4 >> (4 * tc)) & 15) + 1;
+            for (int s = 0; s < tp.nsh[tc]; s++) {
+               const int n = tp.nl[tc][s] * MAXNP;
+               for (int i = tid; i < n; i += NT) {
+                  const int inl = i / MAXNP, ip = i % MAXNP;
+                  co[tp.co_off[tc][s] + i] = ip < npt ? b.data[g.Kof[tc][s] + inl * npt + ip] : 0.0;
+               }
             }
          }
+         cur_g = gi;
       }
-      for (int k = 0; k < tp.nclass; k++) {
-         if (lays[k].skip || lays[k].part != part) continue;
-         if (!setlevel && ks_count(b.ksg[(size_t)g.ks + k].flags) == 0) continue;
-         const CqLayout &y = lays[k];
-         __syncthreads();
-         {
-            const int *src = (const int *)(cps + k);
-            int *dst = (int *)cpl;
-            for (int i = tid; i < (int)(sizeof(ClassPlan) / 4); i += NT) dst[i] = src[i];
-         }
-         __syncthreads();
-         double *WS = y.ws_global ? gws + (size_t)blockIdx.x * gws_stride : lds_base() + y.o_ws;
-         cq_one<NT>(*cpl, k, tp, t, b, kt, y, WS, srcmaps + y.o_src, gi, co, ipb, tabs);
+      const CqLayout &y = lays[k];
+      {
+         const int *src = (const int *)(cps + k);
+         int *dst = (int *)cpl;
+         for (int i = tid; i < (int)(sizeof(ClassPlan) / 4); i += NT) dst[i] = src[i];
       }
       __syncthreads();
+      double *WS = y.ws_global ? gws + (size_t)blockIdx.x * gws_stride : lds_base() + y.o_ws;
+      cq_one<NT>(*cpl, k, tp, t, b, kt, y, WS, srcmaps + y.o_src, gi, co, ipb, tabs);
    }
 }
 
@@ -1262,32 +1298,48 @@ void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::ve
    for (int k = 0; k < tp.nclass; k++) {
       if (tq_has(tp.cls_code[k])) continue;
       const int need = 3 * cps[k].ycap + cps[k].Nc + CQ_PCH + tp.cls_nl[k] * cps[k].Ns + CQ_PCH;
-      budget = std::max(budget, std::min(need, 8192 - 160 - tp.nco_tot - 4 - CQ_TAB_BYTES / 8 - 2));
+      budget = std::max(budget, std::min(need, 8192 - 160 - tp.nco_tot - 4 - CQ_TAB_BYTES / 8 - CQ_PLIST_BYTES / 8 - 2));
    }
    lays.clear();
    srcmaps.clear();
    *lds_bytes = 0;
    *gws_stride = 0;
+   int eco_row = 0;
    for (int k = 0; k < tp.nclass; k++) {
       const ClassPlan &cp = cps[k];
       CqLayout y = cq_layout(cp, tp.cls_nl[k], budget);
       y.skip = tq_has(tp.cls_code[k]) ? 1 : 0;
+      // many contractions: contracted on the matrix cores (k_eco), a row's
+      // contraction buffer holding the class's nq x ncp doubles.  Every
+      // class with at least ECO_CAND_NQ contractions has its place in the
+      // buffer, so that the autotuner can move it there
+      const bool cand = tp.cls_nl[k] >= ECO_CAND_NQ;
+      y.eco = (cand && !y.skip && tp.cls_nl[k] >= ECO_MIN_NQ) ? 1 : 0;
+      y.eco_off = cand ? eco_row : -1;
+      if (cand) eco_row += tp.cls_nl[k] * cp.ncp;
       y.o_src = (int)srcmaps.size();
       for (int e = cp.la; e <= cp.LA; e++)
          for (int f = cp.lc; f <= cp.LC; f++)
             for (int i = 0; i < nco(e) * nco(f); i++) srcmaps.push_back(tp.cpoff[e][f] + i);
-      if (!y.skip) {
-         *lds_bytes = std::max(*lds_bytes, y.bytes);
-         if (y.ws_global) *gws_stride = std::max(*gws_stride, y.lws);
-      }
+      // every class, so that one of the templated kernel's classes can move
+      // to k_gqs
+      *lds_bytes = std::max(*lds_bytes, y.bytes);
+      if (y.ws_global) *gws_stride = std::max(*gws_stride, y.lws);
       y.part = 0;
       lays.push_back(y);
    }
-   // the parts: largest estimated cost first, each to the lightest part
+   assign_parts(tp, cps, lays);
+}
+
+// the k_gqs classes over the GQS_PARTS concurrent launches: largest
+// estimated cost first, each to the lightest part
+void assign_parts(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::vector<CqLayout> &lays)
+{
    std::vector<std::pair<double, int>> cost;
    for (int k = 0; k < tp.nclass; k++)
       if (!lays[k].skip)
-         cost.push_back({(double)tp.cls_nl[k] * (2.0 * cps[k].ncp + cps[k].hrr_flops + cps[k].sph_flops + 8.0 * cps[k].Ns), k});
+         cost.push_back({(double)tp.cls_nl[k] * ((lays[k].eco ? 0.0 : 2.0 * cps[k].ncp) + cps[k].hrr_flops +
+                                                  cps[k].sph_flops + 8.0 * cps[k].Ns), k});
    std::sort(cost.begin(), cost.end(), [](const std::pair<double, int> &a, const std::pair<double, int> &b) {
       return a.first > b.first || (a.first == b.first && a.second < b.second);
    });
@@ -1302,15 +1354,18 @@ void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::ve
 }
 
 int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lays, const int *d_srcmaps, int lds_bytes,
-               int gws_stride, int part, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s,
+               int gws_stride, int part, int npc, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s,
                void *stream)
 {
    if (b.g1 <= b.g0) return 0;
    const int cls_bytes = (lds_bytes + 15) & ~15;
-   const size_t total = (size_t)cls_bytes + CPL_BYTES + 8 * (size_t)tp.nco_tot + 4 * CQ_PCH + 16 + CQ_TAB_BYTES;
+   const size_t total = (size_t)cls_bytes + CPL_BYTES + 8 * (size_t)tp.nco_tot + 4 * CQ_PCH + 16 + CQ_TAB_BYTES +
+                        CQ_PLIST_BYTES;
    if (total > LDS_LIMIT) return -3;
-   const uint32_t ng = b.g1 - b.g0;
-   unsigned grid = std::min(ng, 1u << 20);
+   if (npc <= 0) return 0;
+   const uint64_t nitems = (uint64_t)(b.g1 - b.g0) * (uint64_t)npc;
+   if (nitems > 0xffffffffull) return -4;
+   unsigned grid = (unsigned)std::min<uint64_t>(nitems, GQS_MAX_GRID);
    double *gws = nullptr;
    if (gws_stride) {
       // each part has its own workspace (the parts run concurrently), one
@@ -1359,6 +1414,75 @@ __global__ void k_zero_u32(uint32_t *p, int n)
 {
    const int i = blockIdx.x * blockDim.x + threadIdx.x;
    if (i < n) p[i] = 0u;
+}
+
+// max |P| over n elements into *out (a positive double orders as its bits;
+// *out starts at 0)
+__global__ void k_absmax(const double *__restrict__ P, size_t n, unsigned long long *out)
+{
+   double m = 0.0;
+   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+      m = fmax(m, fabs(P[i]));
+   for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+   if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+// The mixed-precision route of each set quartet of a chunk: single
+// precision when its bound  max|block element| x max|coefficient|^4 x
+// primitive quartets  times max|P| is below tau.  count[0] += flagged.
+__global__ void k_mixed_flags(const TypePlan *__restrict__ tpp, BatchDev b, const double *__restrict__ gmag,
+                              const unsigned long long *__restrict__ pmax, double tau, uint32_t *gflag,
+                              unsigned long long *count)
+{
+   const TypePlan &tp = *tpp;
+   const double maxP = __longlong_as_double((long long)*pmax);
+   for (uint32_t gi = b.g0 + blockIdx.x * blockDim.x + threadIdx.x; gi < b.g1; gi += gridDim.x * blockDim.x) {
+      const GroupRec &g = b.groups[gi];
+      double c4 = 1.0;
+      for (int t = 0; t < 4; t++) {
+         const int np = ((g.np4 >> (4 * t)) & 15) + 1;
+         double cm = 0.0;
+         for (int sh = 0; sh < tp.nsh[t]; sh++)
+            for (int i = 0; i < tp.nl[t][sh] * np; i++) cm = fmax(cm, fabs(b.data[g.Kof[t][sh] + i]));
+         c4 *= cm;
+      }
+      double npr = 0.0;
+      for (uint32_t r = g.row0; r < g.row1; r++) npr += (double)b.rows[r].n_prm;
+      const bool f32 = gmag[gi] * c4 * npr * maxP < tau;
+      gflag[gi] = f32 ? 1u : 0u;
+      if (f32) atomicAdd(count, 1ull);
+   }
+}
+
+int launch_absmax(const double *P, size_t n, unsigned long long *out, bool zero, void *stream)
+{
+   if (zero) hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, (hipStream_t)stream, (uint32_t *)out, 2);
+   const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 2048);
+   hipLaunchKernelGGL(k_absmax, dim3(std::max(grid, 1u)), dim3(256), 0, (hipStream_t)stream, P, n, out);
+   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_mixed_flags(const TypePlan &tp, const BatchDev &b, const double *gmag, const unsigned long long *pmax,
+                       double tau, uint32_t *gflag, unsigned long long *count, void *stream)
+{
+   (void)tp;
+   if (b.g1 <= b.g0) return 0;
+   const unsigned grid = std::min<unsigned>((b.g1 - b.g0 + 255) / 256, 4096);
+   hipLaunchKernelGGL(k_mixed_flags, dim3(grid), dim3(256), 0, (hipStream_t)stream, b.tp, b, gmag, pmax, tau, gflag,
+                      count);
+   return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void k_zero_f64(double *p, size_t n)
+{
+   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+void launch_zero_f64(double *p, size_t n, void *stream)
+{
+   if (!n) return;
+   const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+   hipLaunchKernelGGL(k_zero_f64, dim3(grid), dim3(256), 0, (hipStream_t)stream, p, n);
 }
 
 int launch_zero_u32(uint32_t *p, int n, void *stream)

@@ -376,3 +376,103 @@ def stats(lib, prefix, h):
 
 def product():
     return load()
+
+
+# ---------------------------------------------------------------------------
+# serving workload: independent periodic systems sending K-build requests to
+# one device-resident request loop (libgint_amd_server_*)
+# ---------------------------------------------------------------------------
+WATER_VOLUME_A3 = 29.9   # A^3 per water molecule at 1 g/cm^3
+
+
+def serving_systems(n_systems=16, atoms=(20, 200), seed=3, image_shells=1, np_max=3, zmin=0.6):
+    """n_systems periodic water-like systems, sizes log-uniform in `atoms`,
+    each in a cubic box at liquid density, O 3s2p1d / H 2s1p with 1..np_max
+    primitives (exponents log-uniform in [zmin, 5e3])"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_systems):
+        a = float(np.exp(rng.uniform(np.log(atoms[0]), np.log(atoms[1]))))
+        nw = max(1, int(round(a / 3)))
+        box = (nw * WATER_VOLUME_A3) ** (1.0 / 3.0)
+        out.append(System(n_water=nw, box_A=box, image_shells=image_shells, basis=0, np_min=1, np_max=np_max,
+                          zmin=zmin, seed=1000 * seed + i))
+    return out
+
+
+class ServingClient:
+    """one system as a client of the server: its plan recorded into a handle
+    of its own, its SCF-like density sequence"""
+
+    def __init__(self, lib, system, steps=10, seed=0):
+        self.s = system
+        self.st = system.record()
+        self.P0 = system.P()
+        self.h, table = setup_handle(lib, "libgint_", system, P=self.P0, max_mem_mib=1024)
+        lib.libgint_amd_plan_begin(self.h)
+        self.st.replay(table, self.h, 0, self.st.n_sets, 1, True)
+        lib.libgint_amd_plan_end(self.h)
+        self.rng = np.random.default_rng(seed)
+        self.steps = steps
+        self.k = 0
+        self.P = self.P0.copy()
+
+    def next_density(self):
+        """the density of the next SCF-like step: the previous one plus
+        1e-3 N(0,1), back to the start after `steps` steps"""
+        if self.k % self.steps == 0:
+            self.P = self.P0.copy()
+        else:
+            self.P = self.P + 1e-3 * self.rng.standard_normal(self.P.size)
+        self.k += 1
+        return self.P.copy()
+
+
+def serve(lib, clients, n_requests, rate, slots, max_mem_mib, seed=7, keep=False, resident_clients=None):
+    """Poisson arrivals at `rate` requests/s, each to a client drawn at
+    random, through one server; returns per-request (client, P, K or None,
+    rc, latency, queueing, execution) and the server statistics.  With
+    resident_clients = n the budget is set to hold the n largest clients"""
+    import threading
+    import time
+
+    srv = lib.libgint_amd_server_create(max_mem_mib, slots)
+    ids = [lib.libgint_amd_server_add_client(srv, c.h, c.s.P_size) for c in clients]
+    if resident_clients:
+        sizes = sorted((lib.libgint_amd_server_client_bytes(srv, i) for i in ids), reverse=True)
+        lib.libgint_amd_server_set_budget(srv, int(sum(sizes[:resident_clients]) / 2**20) + 1)
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n_requests)
+    who = rng.integers(0, len(clients), n_requests)
+    reqs = [None] * n_requests
+    done = [None] * n_requests
+
+    def waiter():
+        out = np.zeros(3)
+        for i in range(n_requests):
+            while reqs[i] is None:
+                time.sleep(0.0005)
+            rid, c, P, K = reqs[i]
+            rc = lib.libgint_amd_server_wait(srv, rid, _ptr(out))
+            done[i] = (c, P if keep else None, K if keep else None, rc, float(out[0]), float(out[1]), float(out[2]))
+            if not keep:
+                reqs[i] = (rid, c, None, None)   # the server is done with P and K
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = 0.0
+    for i in range(n_requests):
+        t_next += gaps[i]
+        c = int(who[i])
+        P = clients[c].next_density()
+        K = np.zeros(clients[c].s.P_size)
+        while time.perf_counter() - t0 < t_next:
+            time.sleep(0.0002)
+        rid = lib.libgint_amd_server_submit(srv, ids[c], _ptr(P), _ptr(K))
+        reqs[i] = (rid, c, P, K)
+    th.join()
+    st = np.zeros(8)
+    lib.libgint_amd_server_stats(srv, _ptr(st))
+    lib.libgint_amd_server_destroy(srv)
+    return done, st

@@ -23,7 +23,7 @@ REF_SRC = "/root/reference/src"
 def _declared():
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return re.findall(r"^\s*(?:void|double|int)\s*\*?\s*(\w+)\s*\(", txt, flags=re.M)
+    return re.findall(r"^\s*(?:void|double|int64_t|int)\s*\*?\s*(\w+)\s*\(", txt, flags=re.M)
 
 
 def test_exports_every_declared_symbol():
