@@ -141,6 +141,8 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0, help="serving workload: mean arrivals per second")
     ap.add_argument("--slots", type=int, default=4, help="serving workload: concurrent requests")
     ap.add_argument("--budget", type=int, default=7000, help="serving workload: device budget, MiB")
+    ap.add_argument("--roof-only", action="store_true",
+                    help="only the roofline's serialised sample run (for kernel-trace and counter profiles)")
     ap.add_argument("--no-progress", dest="progress", action="store_false",
                     help="no per-quartet-type progress lines (and no host wait per type) in the steps")
     a = ap.parse_args()
@@ -196,86 +198,100 @@ def main():
                          f"scaled to {n_sets}"}
         log(f"ABI ingestion: {abi['sample']} -> {abi['s_per_step_est']} s per step on one thread")
 
-    h, table = D.setup_handle(lib, "libgint_", s, P=P, max_mem_mib=a.max_mem, c0=c0)
-    K = np.zeros(s.P_size)
-    t0 = time.time()
-    lib.libgint_amd_plan_begin(h)
-    nsl = 8
-    for i in range(nsl):
-        st.replay(table, h, st.n_sets * i // nsl, st.n_sets * (i + 1) // nsl, a.stride, True)
-        log(f"ingested {i + 1}/{nsl} ({time.time() - t0:.1f}s)")
-    lib.libgint_amd_plan_end(h)
-    t_ing = time.time() - t0
-    info = (C.c_uint64 * 6)()
-    lib.libgint_amd_plan_info(h, info)
-    log(f"set-level ingestion {t_ing:.1f}s, resident plan {info[0] / 2**30:.2f} GiB, {info[1]} quartet types, "
-        f"{info[2]} chunks")
-    # debug bit 2: the engine waits for each quartet type's work and reports
-    # it (progress of long K-builds; one host wait per type, ~40 per step)
-    lib.libgint_amd_set_debug(h, 4 if a.progress else 0)
+    def main_steps():
+        h, table = D.setup_handle(lib, "libgint_", s, P=P, max_mem_mib=a.max_mem, c0=c0)
+        K = np.zeros(s.P_size)
+        t0 = time.time()
+        lib.libgint_amd_plan_begin(h)
+        nsl = 8
+        for i in range(nsl):
+            st.replay(table, h, st.n_sets * i // nsl, st.n_sets * (i + 1) // nsl, a.stride, True)
+            log(f"ingested {i + 1}/{nsl} ({time.time() - t0:.1f}s)")
+        lib.libgint_amd_plan_end(h)
+        t_ing = time.time() - t0
+        info = (C.c_uint64 * 6)()
+        lib.libgint_amd_plan_info(h, info)
+        log(f"set-level ingestion {t_ing:.1f}s, resident plan {info[0] / 2**30:.2f} GiB, {info[1]} quartet types, "
+            f"{info[2]} chunks")
+        # debug bit 2: the engine waits for each quartet type's work and reports
+        # it (progress of long K-builds; one host wait per type, ~40 per step)
+        lib.libgint_amd_set_debug(h, 4 if a.progress else 0)
 
-    def step(PP=P, out=K):
-        lib.libgint_set_P(h, dp(PP), s.P_size)
-        lib.libgint_amd_plan_execute(h)
-        lib.libgint_get_K(h, dp(out))
+        def step(PP=P, out=K):
+            lib.libgint_set_P(h, dp(PP), s.P_size)
+            lib.libgint_amd_plan_execute(h)
+            lib.libgint_get_K(h, dp(out))
 
-    # warm-up steps count the surviving slots (device counters, untimed)
-    lib.libgint_amd_enable_stats(h, 1)
-    lib.libgint_amd_reset_counters(h)
-    for w in range(a.warmup):
-        t = time.perf_counter()
-        step()
-        log(f"warmup {w} {time.perf_counter() - t:.3f}s")
-    stats = D.stats(lib, "libgint_", h)
-    slots = stats[1] // max(a.warmup, 1)
-    lib.libgint_amd_enable_stats(h, 0)
-    lib.libgint_amd_synchronize(h)
-    times = []
-    for k in range(a.steps):
+        # warm-up steps count the surviving slots (device counters, untimed)
+        lib.libgint_amd_enable_stats(h, 1)
+        lib.libgint_amd_reset_counters(h)
+        for w in range(a.warmup):
+            t = time.perf_counter()
+            step()
+            log(f"warmup {w} {time.perf_counter() - t:.3f}s")
+        stats = D.stats(lib, "libgint_", h)
+        slots = stats[1] // max(a.warmup, 1)
+        lib.libgint_amd_enable_stats(h, 0)
         lib.libgint_amd_synchronize(h)
-        t = time.perf_counter()
-        step()
-        lib.libgint_amd_synchronize(h)
-        times.append(time.perf_counter() - t)
-        log(f"step {k} {times[-1]:.3f}s")
-    tot = sum(times)
-
-    # mixed precision: set quartets whose calibrated bound x max|P| < tau
-    # take the single-precision matrix-core contraction; one FP64 calibration
-    # execution (untimed), then the same number of timed steps, K compared
-    # with the FP64 steps' K of the same density
-    mixed = None
-    if a.mixed > 0 and a.steps:
-        lib.libgint_amd_set_mixed_precision(h, a.mixed)
-        Kc = np.zeros(s.P_size)
-        t = time.perf_counter()
-        step(out=Kc)
-        log(f"mixed precision: calibration step {time.perf_counter() - t:.3f}s")
-        mtimes = []
-        Km = np.zeros(s.P_size)
+        times = []
         for k in range(a.steps):
             lib.libgint_amd_synchronize(h)
             t = time.perf_counter()
-            step(out=Km)
+            step()
             lib.libgint_amd_synchronize(h)
-            mtimes.append(time.perf_counter() - t)
-            log(f"mixed step {k} {mtimes[-1]:.3f}s")
-        mi = (C.c_double * 4)()
-        lib.libgint_amd_mixed_info(h, mi)
-        dk = float(np.max(np.abs(Km - K)))
-        mixed = {"tau": a.mixed, "s_per_step": round(sum(mtimes) / len(mtimes), 4),
-                 "speedup_vs_fp64": round((tot / a.steps) / (sum(mtimes) / len(mtimes)), 4),
-                 "max_abs_dK_vs_fp64": dk, "bar_Ha": 1e-8, "routed_set_quartets": int(mi[2]),
-                 "set_quartets": int(mi[3]),
-                 "path": "single-precision matrix-core contraction of the routed set quartets, the rest FP64"}
-        log(f"mixed precision tau {a.mixed}: {mixed['s_per_step']}s per step (x{mixed['speedup_vs_fp64']}), "
-            f"routed {int(mi[2])} of {int(mi[3])} set quartets, max|dK| vs FP64 {dk:.3e}")
-        lib.libgint_amd_set_mixed_precision(h, 0.0)
-    tu = (C.c_double * 5)()
-    lib.libgint_amd_tune_info(h, tu)
-    tune = {"classes_templated": int(tu[1]), "classes_workgroup": int(tu[2]), "classes_matrix_core": int(tu[3]),
-            "seconds": round(tu[4], 2)}
-    lib.libgint_amd_destroy_handle(h)
+            times.append(time.perf_counter() - t)
+            log(f"step {k} {times[-1]:.3f}s")
+        tot = sum(times)
+
+        # mixed precision: set quartets whose calibrated bound x max|P| < tau
+        # take the single-precision matrix-core contraction; one FP64 calibration
+        # execution (untimed), then the same number of timed steps, K compared
+        # with the FP64 steps' K of the same density
+        mixed = None
+        if a.mixed > 0 and a.steps:
+            lib.libgint_amd_set_mixed_precision(h, a.mixed)
+            Kc = np.zeros(s.P_size)
+            t = time.perf_counter()
+            step(out=Kc)
+            log(f"mixed precision: calibration step {time.perf_counter() - t:.3f}s")
+            mtimes = []
+            Km = np.zeros(s.P_size)
+            for k in range(a.steps):
+                lib.libgint_amd_synchronize(h)
+                t = time.perf_counter()
+                step(out=Km)
+                lib.libgint_amd_synchronize(h)
+                mtimes.append(time.perf_counter() - t)
+                log(f"mixed step {k} {mtimes[-1]:.3f}s")
+            mi = (C.c_double * 4)()
+            lib.libgint_amd_mixed_info(h, mi)
+            dk = float(np.max(np.abs(Km - K)))
+            mixed = {"tau": a.mixed, "s_per_step": round(sum(mtimes) / len(mtimes), 4),
+                     "speedup_vs_fp64": round((tot / a.steps) / (sum(mtimes) / len(mtimes)), 4),
+                     "max_abs_dK_vs_fp64": dk, "bar_Ha": 1e-8, "routed_set_quartets": int(mi[2]),
+                     "set_quartets": int(mi[3]),
+                     "path": "single-precision matrix-core contraction of the routed set quartets, the rest FP64"}
+            log(f"mixed precision tau {a.mixed}: {mixed['s_per_step']}s per step (x{mixed['speedup_vs_fp64']}), "
+                f"routed {int(mi[2])} of {int(mi[3])} set quartets, max|dK| vs FP64 {dk:.3e}")
+            lib.libgint_amd_set_mixed_precision(h, 0.0)
+        tu = (C.c_double * 5)()
+        lib.libgint_amd_tune_info(h, tu)
+        tms = (C.c_double * 256)()
+        tpk = (C.c_int * 256)()
+        lib.libgint_amd_tune_detail(h, tms, tpk)
+        top = sorted(((tms[i], i) for i in range(256) if tms[i] > 0), reverse=True)[:12]
+        log("autotune: classes by sampled time: " + ", ".join(
+            f"({i // 64}{(i // 16) % 4}|{(i // 4) % 4}{i % 4}) {ms:.1f}ms/{'twe'[tpk[i]]}" for ms, i in top))
+        tune = {"classes_templated": int(tu[1]), "classes_workgroup": int(tu[2]), "classes_matrix_core": int(tu[3]),
+                "seconds": round(tu[4], 2)}
+        lib.libgint_amd_destroy_handle(h)
+        return slots, tot, info, t_ing, mixed, tune
+
+    if a.roof_only:
+        slots, tot, info, t_ing, mixed, tune = 0, 0.0, [0] * 6, 0.0, None, None
+        a.steps = 0
+    else:
+        slots, tot, info, t_ing, mixed, tune = main_steps()
 
     # the roofline: a stride sample of the same workload as its own plan,
     # executed once more with every kernel serialised on one stream and
@@ -290,6 +306,11 @@ def main():
     lib.libgint_amd_plan_execute(hr)
     lib.libgint_amd_synchronize(hr)   # first execution: buffers, code objects
     lib.libgint_amd_reset_counters(hr)
+    # a marker launch (the math self-test kernel): in a kernel trace of this
+    # command, the dispatches after it are the timed execution's
+    xm = np.ones(1)
+    om = np.zeros(4)
+    lib.libgint_amd_math_selftest(dp(xm), dp(xm), 1, dp(om))
     Ki = np.zeros(s.P_size)
     t = time.perf_counter()
     lib.libgint_set_P(hr, dp(P), s.P_size)
@@ -304,6 +325,12 @@ def main():
     nsr = len(range(0, st.n_sets, a.stride * rs))
     log(f"roofline sample (every {a.stride * rs}th set quartet, {nsr}): serialised instrumented step {t_ser:.3f}s; "
         f"kernel ms fm_r {tm[0]:.0f} tq {tm[1]:.0f} gqs {tm[2]:.0f} fixup {tm[3]:.0f} eco {te[0]:.0f}")
+    tms = (C.c_double * 256)()
+    tpk = (C.c_int * 256)()
+    lib.libgint_amd_tune_detail(hr, tms, tpk)
+    top = sorted(((tms[i], i) for i in range(256) if tms[i] > 0), reverse=True)[:16]
+    log("roofline sample autotune: classes by sampled time: " + ", ".join(
+        f"({i // 64}{(i // 16) % 4}|{(i // 4) % 4}{i % 4}) {ms:.1f}ms/{'twe'[tpk[i]]}" for ms, i in top))
     for cat, nm in ((0, "fm_r by type L"), (1, "tq by type L"), (2, "gqs by type L"), (4, "eco by type L")):
         det = (C.c_double * 256)()
         lib.libgint_amd_timing_detail(hr, cat, det)

@@ -175,6 +175,10 @@ void libgint_amd_mixed_info(void *handle, double out[4]);
 /* out[0] the plan is tuned, [1..3] classes given the templated kernel /
    the workgroup kernel / the matrix-core contraction, [4] seconds spent */
 void libgint_amd_tune_info(void *handle, double out[5]);
+/* per class code ((la*4+lb)*4+lc)*4+ld: ms[] the chosen schedule's time on
+   the timing samples (summed over quartet types), pick[] that schedule
+   (0 templated, 1 workgroup, 2 matrix cores; last type timed) */
+void libgint_amd_tune_detail(void *handle, double ms[256], int pick[256]);
 /* resident plans: the calls between plan_begin and plan_end are kept on the
    device; plan_execute adds their exchange terms to K for the density of the
    last set_P (between set_P and get_K, in place of the calls) */

@@ -65,6 +65,7 @@ ABI = {
     "libgint_amd_set_mixed_precision": (None, [_vp, C.c_double]),
     "libgint_amd_mixed_info": (None, [_vp, _dp]),
     "libgint_amd_tune_info": (None, [_vp, _dp]),
+    "libgint_amd_tune_detail": (None, [_vp, _dp, C.POINTER(_i)]),
     "libgint_amd_graph_info": (None, [_vp, C.POINTER(C.c_uint64)]),
     "libgint_amd_plan_begin": (None, [_vp]),
     "libgint_amd_plan_end": (None, [_vp]),

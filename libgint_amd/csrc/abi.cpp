@@ -138,6 +138,7 @@ void libgint_amd_set_autotune(void *h, int on) { E(h)->set_autotune(on != 0); }
 void libgint_amd_set_mixed_precision(void *h, double tau) { E(h)->set_mixed_precision(tau); }
 void libgint_amd_mixed_info(void *h, double out[4]) { E(h)->mixed_info(out); }
 void libgint_amd_tune_info(void *h, double out[5]) { E(h)->tune_info(out); }
+void libgint_amd_tune_detail(void *h, double ms[256], int pick[256]) { E(h)->tune_detail(ms, pick); }
 void libgint_amd_graph_info(void *h, uint64_t out[4]) { E(h)->graph_info(out); }
 void libgint_amd_plan_begin(void *h) { E(h)->plan_begin(); }
 void libgint_amd_plan_end(void *h) { E(h)->plan_end(); }

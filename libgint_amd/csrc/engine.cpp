@@ -1728,6 +1728,8 @@ void Engine::autotune(const RunBufs &w, const KTargets &kt)
                }
             }
             dec[k] = bv;
+            tune_ms_[tp.cls_code[k]] += best;
+            tune_pick_[tp.cls_code[k]] = bv;
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tune[tune_key(tp.cls_code[k], tp.cls_nl[k], D.n_prm, D.n_rows)] = bv;
          }

@@ -178,6 +178,14 @@ class Engine {
       for (int i = 0; i < 3; i++) out[1 + i] = (double)tune_counts_[i];
       out[4] = tune_seconds_;
    }
+   // per class code: ms of the chosen schedule on the timing samples, and the schedule
+   void tune_detail(double ms[256], int pick[256]) const
+   {
+      for (int i = 0; i < 256; i++) {
+         ms[i] = tune_ms_[i];
+         pick[i] = tune_pick_[i];
+      }
+   }
    void graph_info(uint64_t out[4]) const
    {
       out[0] = graph_on_ ? 1 : 0;
@@ -324,6 +332,8 @@ class Engine {
    bool plan_tuned_ = false;
    uint64_t tune_counts_[3] = {0, 0, 0};          // classes given k_tqw / k_gqs / k_eco + k_gqs
    double tune_seconds_ = 0.0;
+   double tune_ms_[256] = {};
+   int tune_pick_[256] = {};
 
    // mixed precision (plans): set quartets whose calibrated bound times
    // max |P| is below tau take the single-precision contraction
