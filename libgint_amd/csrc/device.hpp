@@ -325,6 +325,8 @@ struct CqLayout {
    int bytes;
    int eco;         // contracted on the matrix cores (k_eco): k_gqs reads the contraction buffer
    int eco_off;     // the class's offset in a row of the contraction buffer
+   int o_kron;      // LDS of the Kronecker path: bra components x ket spherical pairs (whole 16x16 tiles), T tables
+   int kron_ws;     // doubles of a workgroup's HBM slice that the Kronecker path uses: the two maps of up to 8 rows
 };
 
 // address of the no-allocation scratch array on the current device, looked

@@ -1153,6 +1153,265 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
    }
 }
 
+// ---------------------------------------------------------------------------
+// Kronecker path of k_gqs for classes whose HRR layers do not fit in LDS
+// and whose contraction ran on the matrix cores.  The HRR of a side,
+// (x, y + 1_d| = (x + 1_d, y| + R_d (x y| with R = AB (bra) or CD (ket)
+// (reference src/compute_HRR.cpp), unrolls to
+//    (x y| = sum_{0 <= i <= y} prod_d C(y_d, i_d) R_d^(y_d - i_d) (x + i, 0|,
+// so with the spherical transform (src/compute_SPH.cpp:58-237) folded in,
+// the spherical pair (s1 s2) of a side is a fixed combination of the side's
+// block components g (shells l1 .. l1 + l2, shell outer):
+//    M[s1 s2][g] = sum_x c2s(s1, x) T[s2][g - x],
+//    T[s2][i]    = sum_{y >= i} c2s(s2, y) prod_d C(y_d, i_d) R_d^(y_d - i_d),
+// i running over the vectors with |i| <= l2 (index ioff3(|i|) + cartesian
+// index).  A row's spherical integrals are then M_bra C M_ket^T, C its
+// contracted blocks (bra components x ket components): two small dense
+// products per contraction and row on the matrix cores, summed over the
+// set quartet's rows in the accumulators.
+// ---------------------------------------------------------------------------
+typedef double kdbl4 __attribute__((ext_vector_type(4)));
+
+// v_mfma_f64_16x16x4f64, lane layout as in the contraction kernel: lane l
+// holds A[l % 16][l / 16] and B[l / 16][l % 16]; element r of its D is
+// D[4 r + l / 16][l % 16]
+__device__ __forceinline__ kdbl4 kron_mfma(double a, double b, kdbl4 c)
+{
+   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// binomial coefficient C(n, k), 0 <= k <= n <= 3
+__device__ __forceinline__ double binom3(int n, int k)
+{
+   return (n == 3 && (k == 1 || k == 2)) ? 3.0 : ((n == 2 && k == 1) ? 2.0 : 1.0);
+}
+
+__device__ __forceinline__ double ipow3(double r, int n)
+{
+   return n == 0 ? 1.0 : (n == 1 ? r : (n == 2 ? r * r : r * r * r));
+}
+
+// cartesian components of the shells below k
+__host__ __device__ __forceinline__ int ioff3(int k) { return k * (k + 1) * (k + 2) / 6; }
+
+// T[s2][iv] of a side whose second shell is l2
+__device__ __forceinline__ double kron_t(int l2, int s2, int iv, const double R[3], const double *c2)
+{
+   int K = 0;
+   while (iv >= ioff3(K + 1)) K++;
+   const int ci = iv - ioff3(K);
+   const int i0 = c_cart.comp[K][ci][0], i1 = c_cart.comp[K][ci][1], i2 = c_cart.comp[K][ci][2];
+   const int n2 = d_nco(l2);
+   double v = 0.0;
+   for (int y = 0; y < n2; y++) {
+      const int y0 = c_cart.comp[l2][y][0], y1 = c_cart.comp[l2][y][1], y2 = c_cart.comp[l2][y][2];
+      if (y0 < i0 || y1 < i1 || y2 < i2) continue;
+      v += c2[s2 * n2 + y] * binom3(y0, i0) * binom3(y1, i1) * binom3(y2, i2) * ipow3(R[0], y0 - i0) *
+           ipow3(R[1], y1 - i1) * ipow3(R[2], y2 - i2);
+   }
+   return v;
+}
+
+// M[s][g] of a side (shells l1, l2) from its T table (ni columns)
+__device__ __forceinline__ double kron_m(int l1, int l2, int s, int g, const double *T, int ni, const double *c1)
+{
+   const int ns2 = 2 * l2 + 1;
+   const int s1 = s / ns2, s2 = s - s1 * ns2;
+   int f = l1, gg = g;
+   while (gg >= d_nco(f)) {
+      gg -= d_nco(f);
+      f++;
+   }
+   const int g0 = c_cart.comp[f][gg][0], g1 = c_cart.comp[f][gg][1], g2 = c_cart.comp[f][gg][2];
+   const int K = f - l1, n1 = d_nco(l1);
+   const double *Ts = T + s2 * ni + ioff3(K);
+   double v = 0.0;
+   for (int x = 0; x < n1; x++) {
+      const int i0 = g0 - c_cart.comp[l1][x][0], i1 = g1 - c_cart.comp[l1][x][1], i2 = g2 - c_cart.comp[l1][x][2];
+      if (i0 < 0 || i1 < 0 || i2 < 0) continue;
+      v += c1[s1 * n1 + x] * Ts[(K - i0) * (K - i0 + 1) / 2 + i2];
+   }
+   return v;
+}
+
+// One (set quartet, class) on the Kronecker path, four wavefronts.  LDS at
+// y.o_kron: Y = C M_ket^T [16 TM1][16 TN] (bra components x ket spherical
+// pairs, padded to whole tiles), then the two T tables.  The workgroup's
+// HBM slice W holds the maps of up to kron_ws / (one row's maps) rows:
+// all rows of the set quartet when they fit (built once), else one row at
+// a time (rebuilt per contraction).  Each wavefront keeps a 16-row band of
+// the spherical integrals (bra pairs x ket pairs) in its accumulators over
+// the rows, then makes the contraction's K updates (or writes its
+// integrals) as cq_one does.
+template <int NT>
+__device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b,
+                        const KTargets &kt, const CqLayout &y, double *W, uint32_t gi, const char *tabs)
+{
+   static_assert(NT == 256, "one 16-row band of the integrals per wavefront, four of them");
+   extern __shared__ double lds[];
+   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+   const int la = cp.la, lb = cp.lb, lc = cp.lc, ld = cp.ld;
+   const int ncp = cp.ncp;
+   const int nsa = 2 * la + 1, nsb = 2 * lb + 1, nsc = 2 * lc + 1, nsd = 2 * ld + 1;
+   const int nsab = nsa * nsb, nscd = nsc * nsd, nsbcd = nsb * nscd;
+   int nbra = 0, nket = 0;
+   for (int e = la; e <= cp.LA; e++) nbra += d_nco(e);
+   for (int f = lc; f <= cp.LC; f++) nket += d_nco(f);
+   const int TM1 = (nbra + 15) >> 4, TN = (nscd + 15) >> 4, TM2 = (nsab + 15) >> 4;
+   const int YS = 16 * TN;
+   const int KS1 = (nket + 3) >> 2, KS2 = (nbra + 3) >> 2;
+   const int nik = ioff3(ld + 1), nib = ioff3(lb + 1);
+   double *Y = lds + y.o_kron;
+   double *Tk = Y + 16 * max(TM1, TM2) * YS, *Tb = Tk + nsd * nik;
+   const int MS = nscd * nket + nsab * nbra;
+   const int rc = y.kron_ws / MS;
+   const double *c2s = (const double *)(tabs + CQ_TAB_C2S);
+   const double *ca = c2s + c_c2s_off[la], *cb = c2s + c_c2s_off[lb], *cc = c2s + c_c2s_off[lc],
+                *cd = c2s + c_c2s_off[ld];
+   const int sh = tp.cls_sh[cls_k];
+   const int nlb = tp.nl[1][(sh >> 4) & 15], nlc = tp.nl[2][(sh >> 8) & 15], nld = tp.nl[3][(sh >> 12) & 15];
+   const double symm[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
+   const GroupRec &g = b.groups[gi];
+   const KsGroup ks = group_ks(b, tp, g, cls_k);
+   const int nq = (int)(ks.flags >> 16);
+   if (nq == 0) return;
+   const bool expl = (ks.flags & KS_EXPLICIT) != 0;
+   const bool out_mode = (ks.flags & KS_OUTPUT) != 0;
+   const uint32_t nr = g.row1 - g.row0;
+   const bool cached = nr <= (uint32_t)rc;
+
+   // the two maps of row r into slot m of the slice
+   auto build = [&](uint32_t r, int m) {
+      double AB[3], CD[3];
+      row_ab_cd(g, b.rows[r], b.data, t, AB, CD);
+      __syncthreads();   // the T tables are no longer read
+      for (int idx = tid; idx < nsd * nik + nsb * nib; idx += NT) {
+         if (idx < nsd * nik) {
+            Tk[idx] = kron_t(ld, idx / nik, idx % nik, CD, cd);
+         } else {
+            const int j = idx - nsd * nik;
+            Tb[j] = kron_t(lb, j / nib, j % nib, AB, cb);
+         }
+      }
+      __syncthreads();
+      double *Mk = W + (size_t)m * MS, *Mb = Mk + nscd * nket;
+      for (int idx = tid; idx < MS; idx += NT) {
+         if (idx < nscd * nket) {
+            Mk[idx] = kron_m(lc, ld, idx / nket, idx % nket, Tk, nik, cc);
+         } else {
+            const int j = idx - nscd * nket;
+            Mb[j] = kron_m(la, lb, j / nbra, j % nbra, Tb, nib, ca);
+         }
+      }
+   };
+   if (cached)
+      for (uint32_t r = g.row0; r < g.row1; r++) build(r, (int)(r - g.row0));
+
+   for (int l = 0; l < nq; l++) {
+      kdbl4 acc[4];
+#pragma unroll
+      for (int tn = 0; tn < 4; tn++) acc[tn] = kdbl4{0.0, 0.0, 0.0, 0.0};
+      for (uint32_t r = g.row0; r < g.row1; r++) {
+         if (!cached) build(r, 0);
+         __syncthreads();   // the maps are written, the previous Y is no longer read
+         const double *Mk = W + (size_t)(cached ? r - g.row0 : 0) * MS, *Mb = Mk + nscd * nket;
+         const double *Cl = b.eco + (size_t)(r - b.r0) * b.eco_row + y.eco_off + (size_t)l * ncp;
+         // Y = C M_ket^T, wavefront wv the tiles wv, wv + 4, ... (one
+         // accumulator tile at a time: the integrals' four hold registers)
+         for (int tile = wv; tile < TM1 * TN; tile += 4) {
+            const int tm = tile / TN, tn = tile - tm * TN;
+            const int bc = 16 * tm + (lane & 15), s = 16 * tn + (lane & 15);
+            const bool bv = bc < nbra, sv = s < nscd;
+            int e = la, ie = bc;
+            while (e < cp.LA && ie >= d_nco(e)) {
+               ie -= d_nco(e);
+               e++;
+            }
+            // lane's ket component 4 k4 + lane / 16 as (shell f, index jf)
+            int f = lc, jf = lane >> 4;
+            while (f < cp.LC && jf >= d_nco(f)) {
+               jf -= d_nco(f);
+               f++;
+            }
+            const double *Mks = Mk + s * nket;
+            kdbl4 d = {0.0, 0.0, 0.0, 0.0};
+            for (int gk = lane >> 4; gk < 4 * KS1; gk += 4) {
+               const bool gv = gk < nket;
+               const double av = (bv && gv) ? Cl[cp.cpoff[e][f] + ie * d_nco(f) + jf] : 0.0;
+               const double bm = (gv && sv) ? Mks[gk] : 0.0;
+               d = kron_mfma(av, bm, d);
+               jf += 4;
+               while (f < cp.LC && jf >= d_nco(f)) {
+                  jf -= d_nco(f);
+                  f++;
+               }
+            }
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) Y[(16 * tm + 4 * rr + (lane >> 4)) * YS + 16 * tn + (lane & 15)] = d[rr];
+         }
+         __syncthreads();
+         // integrals += M_bra Y, wavefront wv the 16-row band wv
+         if (wv < TM2) {
+            const int sab = 16 * wv + (lane & 15);
+            for (int k4 = 0; k4 < KS2; k4++) {
+               const int bk = 4 * k4 + (lane >> 4);
+               const double av = (sab < nsab && bk < nbra) ? Mb[sab * nbra + bk] : 0.0;
+#pragma unroll
+               for (int tn = 0; tn < 4; tn++)
+                  if (tn < TN) acc[tn] = kron_mfma(av, Y[bk * YS + 16 * tn + (lane & 15)], acc[tn]);
+            }
+         }
+      }
+      // the integrals of contraction l into LDS (over Y), [bra pair][YS]
+      __syncthreads();
+      if (wv < TM2)
+#pragma unroll
+         for (int tn = 0; tn < 4; tn++)
+            if (tn < TN)
+#pragma unroll
+               for (int rr = 0; rr < 4; rr++)
+                  Y[(16 * wv + 4 * rr + (lane >> 4)) * YS + 16 * tn + (lane & 15)] = acc[tn][rr];
+      __syncthreads();
+      // its K updates (or its integrals, in the TRA order)
+      size_t off[4] = {0, 0, 0, 0};
+      uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
+      if (expl) {
+         const KsRec x = b.ksx[ks.expl + l];
+         for (int q = 0; q < 4; q++) off[q] = x.off[q];
+         ld01 = x.ld01;
+         ld23 = x.ld23;
+         flags = x.flags;
+      } else if (!out_mode) {
+         ks_offsets(ks, nlb, nlc, nld, l, off);
+      }
+      const double fac = symm[(flags >> 4) & 7] * kt.hf_fac;
+      const unsigned ldac = ld01 & 0xffff, ldad = ld01 >> 16, ldbc = ld23 & 0xffff, ldbd = ld23 >> 16;
+      for (int tt = tid; tt < cp.Ns; tt += NT) {
+         const unsigned isa = tt / nsbcd, isb = (tt / nscd) % nsb, isc = (tt / nsd) % nsc, isd = tt % nsd;
+         const double v = Y[(tt / nscd) * YS + tt % nscd];
+         if (out_mode) {
+            double *O = kt.out + ((size_t)ks.base[0] | (size_t)ks.base[1] << 32);
+            const int ia = l / (nlb * nlc * nld), ib = (l / (nlc * nld)) % nlb, ic = (l / nld) % nlc, id = l % nld;
+            O[tra_index(ia, isa, ib, isb, ic, isc, id, isd, nlb, nlc, nld, nsa, nsb, nsc, nsd)] = v;
+            continue;
+         }
+         const size_t Iac = off[0] + blk_idx(flags & 1, ldac, isa, isc);
+         const size_t Iad = off[1] + blk_idx(flags & 2, ldad, isa, isd);
+         const size_t Ibc = off[2] + blk_idx(flags & 4, ldbc, isb, isc);
+         const size_t Ibd = off[3] + blk_idx(flags & 8, ldbd, isb, isd);
+         const double iabcd = -fac * v;
+         for (int sp = 0; sp < kt.nspin; sp++) {
+            const double *P = kt.P[sp];
+            double *K = kt.K[sp];
+            unsafeAtomicAdd(&K[Ibd], iabcd * P[Iac]);
+            unsafeAtomicAdd(&K[Ibc], iabcd * P[Iad]);
+            unsafeAtomicAdd(&K[Iad], iabcd * P[Ibc]);
+            unsafeAtomicAdd(&K[Iac], iabcd * P[Ibd]);
+         }
+      }
+   }
+}
+
 // Work items are (set quartet, class of this part) pairs, the classes of a
 // set quartet on consecutive items: a type with few, large set quartets (an
 // spdf centre's 16^4 primitive quartets and 256 classes) still fills the
@@ -1218,7 +1477,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       }
       __syncthreads();
       double *WS = y.ws_global ? gws + (size_t)blockIdx.x * gws_stride : lds_base() + y.o_ws;
-      cq_one<NT>(*cpl, k, tp, t, b, kt, y, WS, srcmaps + y.o_src, gi, co, ipb, tabs);
+      // the Kronecker path where it applies: contraction on the matrix
+      // cores, layers beyond LDS, and no class-dependent screening to add
+      bool kron = false;
+      if constexpr (NT == 256) kron = y.eco && y.kron_ws && min(b.fix_count[0], b.fix_cap) == 0;
+      if (kron)
+         cq_kron<256>(*cpl, k, tp, t, b, kt, y, WS, gi, tabs);
+      else
+         cq_one<NT>(*cpl, k, tp, t, b, kt, y, WS, srcmaps + y.o_src, gi, co, ipb, tabs);
    }
 }
 
@@ -1286,6 +1552,23 @@ static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget)
    y.o_I = off;
    if (y.reduce) off += need_I;
    y.o_src = 0;
+   // the Kronecker path (cq_kron) of a class whose layers do not fit: one
+   // contraction's bra components x ket spherical functions in LDS, the
+   // row's two maps in the workgroup's HBM slice
+   int nbra = 0, nket = 0;
+   for (int e = cp.la; e <= cp.LA; e++) nbra += nco(e);
+   for (int f = cp.lc; f <= cp.LC; f++) nket += nco(f);
+   const int nsab = (2 * cp.la + 1) * (2 * cp.lb + 1), nscd = (2 * cp.lc + 1) * (2 * cp.ld + 1);
+   const int kron_lds = 16 * std::max((nbra + 15) / 16, (nsab + 15) / 16) * 16 * ((nscd + 15) / 16) +
+                        (2 * cp.ld + 1) * ioff3(cp.ld + 1) + (2 * cp.lb + 1) * ioff3(cp.lb + 1);
+   const int maps = nscd * nket + nsab * nbra;
+   y.o_kron = off;
+   y.kron_ws = 0;
+   if (y.ws_global && off + kron_lds <= budget) {
+      off += kron_lds;
+      // the maps of up to 8 rows, within the slice the layered path uses
+      y.kron_ws = std::max(1, std::min(8, y.lws / maps)) * maps;
+   }
    y.bytes = 8 * off;
    return y;
 }
@@ -1324,7 +1607,7 @@ void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::ve
       // every class, so that one of the templated kernel's classes can move
       // to k_gqs
       *lds_bytes = std::max(*lds_bytes, y.bytes);
-      if (y.ws_global) *gws_stride = std::max(*gws_stride, y.lws);
+      if (y.ws_global) *gws_stride = std::max(*gws_stride, std::max(y.lws, y.kron_ws));
       y.part = 0;
       lays.push_back(y);
    }
@@ -1376,8 +1659,9 @@ int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lay
       gws = s.ptr + (size_t)part * gws_stride * gcap;
    }
    // a type whose workspace leaves room for few workgroups per CU gets wider
-   // ones, so that more wavefronts hide the memory latency
-   if (total > 32 * 1024)
+   // ones, so that more wavefronts hide the memory latency; so does one with
+   // a workspace in HBM (the Kronecker path takes four wavefronts)
+   if (total > 32 * 1024 || gws_stride > 0)
       hipLaunchKernelGGL(k_gqs<256>, dim3(grid), dim3(256), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
                          b.tp, t, b, kt, gws, gws_stride, cls_bytes, part);
    else
