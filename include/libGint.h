@@ -137,7 +137,10 @@ void libgint_amd_timing_detail(void *handle, int cat, double out[256]);
    kernel of a chunk on the handle's stream, one after the other (per-kernel
    durations without overlap, for profiling; results are unchanged); bit 2:
    wait for the work of each quartet type and report its completion on
-   stderr (progress of long K-builds; results are unchanged) */
+   stderr (progress of long K-builds; results are unchanged); bit 3: resident
+   plans send every class that can take it to the matrix-core contraction
+   with the Kronecker HRR, whatever the autotuner chose (a test of that
+   path; results are unchanged) */
 void libgint_amd_set_debug(void *handle, int flags);
 /* libgint_amd_plan_execute replays the plan's launches (every chunk's
    kernels on the handle's stream and its side streams) as one HIP graph:
@@ -173,11 +176,13 @@ void libgint_amd_set_mixed_precision(void *handle, double tau);
    by the last execution, [3] set quartets of the plan */
 void libgint_amd_mixed_info(void *handle, double out[4]);
 /* out[0] the plan is tuned, [1..3] classes given the templated kernel /
-   the workgroup kernel / the matrix-core contraction, [4] seconds spent */
+   the workgroup kernel / the matrix-core contraction (with either HRR),
+   [4] seconds spent */
 void libgint_amd_tune_info(void *handle, double out[5]);
 /* per class code ((la*4+lb)*4+lc)*4+ld: ms[] the chosen schedule's time on
    the timing samples (summed over quartet types), pick[] that schedule
-   (0 templated, 1 workgroup, 2 matrix cores; last type timed) */
+   (0 templated, 1 workgroup, 2 matrix-core contraction + layered HRR,
+   3 matrix-core contraction + Kronecker HRR; last type timed) */
 void libgint_amd_tune_detail(void *handle, double ms[256], int pick[256]);
 /* resident plans: the calls between plan_begin and plan_end are kept on the
    device; plan_execute adds their exchange terms to K for the density of the
@@ -268,6 +273,10 @@ void libgint_amd_server_destroy(void *server);
    request, estimated before), and a new budget (MiB) */
 int64_t libgint_amd_server_client_bytes(void *server, int client);
 void libgint_amd_server_set_budget(void *server, int max_mem_MiB);
+/* the loop's state now: out[0] requests queued, [1] running, [2] done,
+   [3] refused, [4] resident clients, [5] busy clients, [6] resident bytes,
+   [7] budget bytes (for monitoring a serving process) */
+void libgint_amd_server_state(void *server, int64_t out[8]);
 /* reads CP2K's truncated-Coulomb coefficient file (the T_C_G_DATA
    t_c_g.dat) as reference src/t_c_g0_n.cpp:29-51 (read_c0) does: 207
    patches of 22 x 105 white-space separated numbers, patch after patch.

@@ -87,6 +87,7 @@ ABI = {
     "libgint_amd_server_destroy": (None, [_vp]),
     "libgint_amd_server_client_bytes": (C.c_int64, [_vp, _i]),
     "libgint_amd_server_set_budget": (None, [_vp, _i]),
+    "libgint_amd_server_state": (None, [_vp, C.POINTER(C.c_int64)]),
     "libgint_amd_read_c0": (_i, [C.c_char_p, _i, _dp, _i, C.POINTER(_i)]),
     "libgint_amd_write_c0": (_i, [C.c_char_p, _dp, _i]),
 }

@@ -326,7 +326,9 @@ struct CqLayout {
    int eco;         // contracted on the matrix cores (k_eco): k_gqs reads the contraction buffer
    int eco_off;     // the class's offset in a row of the contraction buffer
    int o_kron;      // LDS of the Kronecker path: bra components x ket spherical pairs (whole 16x16 tiles), T tables
-   int kron_ws;     // doubles of a workgroup's HBM slice that the Kronecker path uses: the two maps of up to 8 rows
+   int kron_ws;     // doubles of the Kronecker path's row maps (up to 8 rows); 0: the class cannot take the path
+   int o_kmap;      // LDS offset of those maps, or -1: in the workgroup's HBM slice
+   int kron;        // the class takes the Kronecker path (with eco; a schedule of the autotuner)
 };
 
 // address of the no-allocation scratch array on the current device, looked
@@ -357,7 +359,7 @@ constexpr int GQS_PARTS = 4;
 constexpr unsigned GQS_MAX_GRID = 8192;
 int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lays, const int *d_srcmaps, int lds_bytes,
                int gws_stride, int part, int npc, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s,
-               void *stream);
+               void *stream, bool wide = false);
 // contraction of primitive blocks on the matrix cores for the classes with
 // CqLayout::eco (kernels_eco.hip): k_gqs classes with at least this many
 // contractions

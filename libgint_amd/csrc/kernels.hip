@@ -1236,8 +1236,9 @@ __device__ __forceinline__ double kron_m(int l1, int l2, int s, int g, const dou
 
 // One (set quartet, class) on the Kronecker path, four wavefronts.  LDS at
 // y.o_kron: Y = C M_ket^T [16 TM1][16 TN] (bra components x ket spherical
-// pairs, padded to whole tiles), then the two T tables.  The workgroup's
-// HBM slice W holds the maps of up to kron_ws / (one row's maps) rows:
+// pairs, padded to whole tiles), then the two T tables.  W (the
+// workgroup's HBM slice, or LDS at y.o_kmap for a class whose layered
+// workspace fits in LDS) holds the maps of up to kron_ws / (one row's maps) rows:
 // all rows of the set quartet when they fit (built once), else one row at
 // a time (rebuilt per contraction).  Each wavefront keeps a 16-row band of
 // the spherical integrals (bra pairs x ket pairs) in its accumulators over
@@ -1480,9 +1481,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       // the Kronecker path where it applies: contraction on the matrix
       // cores, layers beyond LDS, and no class-dependent screening to add
       bool kron = false;
-      if constexpr (NT == 256) kron = y.eco && y.kron_ws && min(b.fix_count[0], b.fix_cap) == 0;
+      if constexpr (NT == 256) kron = y.kron && y.eco && y.kron_ws && min(b.fix_count[0], b.fix_cap) == 0;
       if (kron)
-         cq_kron<256>(*cpl, k, tp, t, b, kt, y, WS, gi, tabs);
+         cq_kron<256>(*cpl, k, tp, t, b, kt, y, y.o_kmap >= 0 ? lds_base() + y.o_kmap : WS, gi, tabs);
       else
          cq_one<NT>(*cpl, k, tp, t, b, kt, y, WS, srcmaps + y.o_src, gi, co, ipb, tabs);
    }
@@ -1559,17 +1560,30 @@ static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget)
    for (int e = cp.la; e <= cp.LA; e++) nbra += nco(e);
    for (int f = cp.lc; f <= cp.LC; f++) nket += nco(f);
    const int nsab = (2 * cp.la + 1) * (2 * cp.lb + 1), nscd = (2 * cp.lc + 1) * (2 * cp.ld + 1);
+   // (the layered path's LDS is free while a class takes this one: its
+   // region starts at 0)
    const int kron_lds = 16 * std::max((nbra + 15) / 16, (nsab + 15) / 16) * 16 * ((nscd + 15) / 16) +
                         (2 * cp.ld + 1) * ioff3(cp.ld + 1) + (2 * cp.lb + 1) * ioff3(cp.lb + 1);
    const int maps = nscd * nket + nsab * nbra;
-   y.o_kron = off;
+   y.o_kron = 0;
    y.kron_ws = 0;
-   if (y.ws_global && off + kron_lds <= budget) {
-      off += kron_lds;
-      // the maps of up to 8 rows, within the slice the layered path uses
-      y.kron_ws = std::max(1, std::min(8, y.lws / maps)) * maps;
+   y.o_kmap = -1;
+   y.kron = 0;
+   int kbytes = 0;
+   if (y.ws_global) {
+      // the maps of up to 8 rows in the workgroup's HBM slice, within the
+      // slice the layered path uses
+      if (kron_lds <= budget) {
+         y.kron_ws = std::max(1, std::min(8, y.lws / maps)) * maps;
+         kbytes = kron_lds;
+      }
+   } else if (kron_lds + maps <= budget) {
+      // the maps of up to 8 rows in LDS, within the type's budget
+      y.o_kmap = kron_lds;
+      y.kron_ws = std::min(8, (budget - kron_lds) / maps) * maps;
+      kbytes = kron_lds + y.kron_ws;
    }
-   y.bytes = 8 * off;
+   y.bytes = 8 * std::max(off, kbytes);
    return y;
 }
 
@@ -1607,6 +1621,9 @@ void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::ve
       // every class, so that one of the templated kernel's classes can move
       // to k_gqs
       *lds_bytes = std::max(*lds_bytes, y.bytes);
+      // the Kronecker path by default where the layered one has its
+      // workspace in HBM (the autotuner times both)
+      y.kron = (y.eco && y.ws_global && y.kron_ws) ? 1 : 0;
       if (y.ws_global) *gws_stride = std::max(*gws_stride, std::max(y.lws, y.kron_ws));
       y.part = 0;
       lays.push_back(y);
@@ -1638,7 +1655,7 @@ void assign_parts(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::ve
 
 int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lays, const int *d_srcmaps, int lds_bytes,
                int gws_stride, int part, int npc, const DevTables &t, const BatchDev &b, const KTargets &kt, Scratch s,
-               void *stream)
+               void *stream, bool wide)
 {
    if (b.g1 <= b.g0) return 0;
    const int cls_bytes = (lds_bytes + 15) & ~15;
@@ -1660,8 +1677,9 @@ int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lay
    }
    // a type whose workspace leaves room for few workgroups per CU gets wider
    // ones, so that more wavefronts hide the memory latency; so does one with
-   // a workspace in HBM (the Kronecker path takes four wavefronts)
-   if (total > 32 * 1024 || gws_stride > 0)
+   // a workspace in HBM or a class on the Kronecker path (`wide`: that path
+   // takes four wavefronts)
+   if (total > 32 * 1024 || gws_stride > 0 || wide)
       hipLaunchKernelGGL(k_gqs<256>, dim3(grid), dim3(256), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
                          b.tp, t, b, kt, gws, gws_stride, cls_bytes, part);
    else

@@ -18,7 +18,14 @@
 // wavefront, all accumulated in registers) and walks the row's primitive
 // quartets 16 at a time: their coefficient products are formed once in LDS
 // and serve the 4 wavefronts; each lane loads one block element per 4
-// primitive quartets (16 consecutive columns per primitive quartet).
+// primitive quartets (16 consecutive columns per primitive quartet), the
+// next batch's while the current one runs on the matrix cores.
+//
+// A row of a large type (an spdf quartet: thousands of primitive quartets x
+// 7056 columns) is far larger than the caches, and the classes share its
+// columns, so the items are ordered by the first column they read and an
+// XCD takes a contiguous run of them: items that read the same columns run
+// together, through one L2.
 #include <hip/hip_runtime.h>
 
 #include "device.hpp"
@@ -91,16 +98,50 @@ constexpr int ECO_PB = 16;   // primitive quartets per LDS batch of coefficient 
 constexpr int ECO_QB = 64;   // contractions per work item
 constexpr int ECO_JB = 64;   // columns per work item (4 wavefronts x 16)
 
-__global__ __launch_bounds__(256) void k_eco(const TypePlan *__restrict__ tpp, BatchDev b, EcoArgs e)
+// one batch of ECO_PB primitive quartets into the wavefront's NQT tiles of
+// 16 contractions: a[kk] the lane's block element of primitive quartet
+// 4 kk + lane / 16, Wl = &W[lane / 16][lane % 16].  Single precision (f32)
+// sums the batch in a single-precision tile and adds it to the double
+// accumulators.
+template <int NQT>
+__device__ __forceinline__ void eco_batch(const double *a, const double *Wl, dbl4 *acc, bool f32)
+{
+   if (f32) {
+      flt4 s32[NQT];
+#pragma unroll
+      for (int t = 0; t < NQT; t++) s32[t] = flt4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kk = 0; kk < ECO_PB / 4; kk++)
+#pragma unroll
+         for (int t = 0; t < NQT; t++) s32[t] = mfma_f32_16x16x4((float)a[kk], (float)Wl[4 * kk * ECO_QB + 16 * t], s32[t]);
+#pragma unroll
+      for (int t = 0; t < NQT; t++)
+#pragma unroll
+         for (int rr = 0; rr < 4; rr++) acc[t][rr] += (double)s32[t][rr];
+   } else {
+#pragma unroll
+      for (int kk = 0; kk < ECO_PB / 4; kk++)
+#pragma unroll
+         for (int t = 0; t < NQT; t++) acc[t] = mfma_f64_16x16x4(a[kk], Wl[4 * kk * ECO_QB + 16 * t], acc[t]);
+   }
+}
+
+// three wavefronts per SIMD: the 4 x 16 double accumulators, two batches of
+// block elements and the tile operands fit in 168 registers without spills
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_eco(const TypePlan *__restrict__ tpp, BatchDev b, EcoArgs e)
 {
    const TypePlan &tp = *tpp;
    __shared__ double co[4][MAXNL][MAXNP];   // the class's four shells' coefficients
    __shared__ double W[ECO_PB][ECO_QB];     // coefficient products of the batch
-   __shared__ uint32_t ipl[ECO_PB];
    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
    const uint32_t nrow = b.r1 - b.r0;
    const uint64_t nitems = (uint64_t)nrow * (uint64_t)e.n_items;
-   for (uint64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+   // XCD-aware order: workgroups go to the 8 XCDs round robin, so XCD x
+   // takes the x-th eighth of each grid's worth of items, a contiguous run
+   // that reads the same columns (the items are ordered by column) through
+   // its own L2
+   const uint32_t wg = (gridDim.x & 7) ? blockIdx.x : (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+   for (uint64_t it = wg; it < nitems; it += gridDim.x) {
       const uint32_t r = b.r0 + (uint32_t)(it / (uint64_t)e.n_items);
       const EcoItem item = e.items[it % (uint64_t)e.n_items];
       const int k = item.k;
@@ -111,7 +152,7 @@ __global__ __launch_bounds__(256) void k_eco(const TypePlan *__restrict__ tpp, B
       const int nlb = tp.nl[1][shv[1]], nlc = tp.nl[2][shv[2]], nld = tp.nl[3][shv[3]];
       const int nq = tp.cls_nl[k], cols = tp.cls_ncp[k];
       const int ncpt = tp.ncp;
-      __syncthreads();   // the previous item no longer reads co, W, ipl
+      __syncthreads();   // the previous item no longer reads co, W
       for (int i = tid; i < 4 * MAXNL * MAXNP; i += 256) {
          const int X = i / (MAXNL * MAXNP), inl = (i / MAXNP) % MAXNL, ip = i % MAXNP;
          const int np = ((g.np4 >> (4 * X)) & 15) + 1;
@@ -121,9 +162,15 @@ __global__ __launch_bounds__(256) void k_eco(const TypePlan *__restrict__ tpp, B
       const int jt = item.jt0 + wv;
       const int j = jt * 16 + (lane & 15);
       const bool jv = j < cols;
+      const bool tv = jt * 16 < cols;
       const int src = jv ? e.srcmaps[e.src_off[k] + j] : 0;
       const int q0 = item.qb * ECO_QB;
       const int nqt = min(4, (nq - q0 + 15) / 16);
+      // this thread's coefficient products: contraction q0 + (tid & 63) for
+      // primitive quartets tid / 64 + 4 m of each batch
+      const int qq = tid & 63, q = q0 + qq;
+      const bool qv = q < nq;
+      const int ia = q / (nlb * nlc * nld), ib = (q / (nlc * nld)) % nlb, ic = (q / nld) % nlc, id = q % nld;
       // mixed precision: single-precision tiles for a set quartet whose
       // bound is small (each batch of 16 primitive quartets summed in single
       // precision, the batches in double precision)
@@ -131,69 +178,68 @@ __global__ __launch_bounds__(256) void k_eco(const TypePlan *__restrict__ tpp, B
       dbl4 acc[4];
 #pragma unroll
       for (int t = 0; t < 4; t++) acc[t] = dbl4{0.0, 0.0, 0.0, 0.0};
-      const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * ncpt;
+      const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * ncpt + src;
       const uint32_t *IP = b.prm_ip + row.prm_off;
       const int n = (int)row.n_prm;
+      // the next batch's block elements and primitive indices, loaded while
+      // the current batch runs
+      double an[ECO_PB / 4];
+      uint32_t ipn[ECO_PB / 4];
+#pragma unroll
+      for (int kk = 0; kk < ECO_PB / 4; kk++) {
+         const int p = 4 * kk + (lane >> 4);
+         an[kk] = (jv && p < n) ? A0[(size_t)p * ncpt] : 0.0;
+         const int pp = (tid >> 6) + 4 * kk;
+         ipn[kk] = pp < n ? IP[pp] : 0u;
+      }
       for (int pb = 0; pb < n; pb += ECO_PB) {
-         __syncthreads();
-         if (tid < ECO_PB) ipl[tid] = pb + tid < n ? IP[pb + tid] : 0u;
-         __syncthreads();
-         for (int i = tid; i < ECO_PB * ECO_QB; i += 256) {
-            const int pp = i / ECO_QB, qq = i % ECO_QB, q = q0 + qq;
-            double w = 0.0;
-            if (pb + pp < n && q < nq) {
-               const uint32_t ip = ipl[pp];
-               const int ia = q / (nlb * nlc * nld), ib = (q / (nlc * nld)) % nlb, ic = (q / nld) % nlc, id = q % nld;
-               w = co[0][ia][ip & 15] * co[1][ib][(ip >> 4) & 15] * co[2][ic][(ip >> 8) & 15] *
-                   co[3][id][(ip >> 12) & 15];
-            }
-            W[pp][qq] = w;
+         double a[ECO_PB / 4];
+         uint32_t ipc[ECO_PB / 4];
+#pragma unroll
+         for (int kk = 0; kk < ECO_PB / 4; kk++) {
+            a[kk] = an[kk];
+            ipc[kk] = ipn[kk];
+         }
+         __syncthreads();   // the previous batch's W is no longer read
+#pragma unroll
+         for (int m = 0; m < ECO_PB / 4; m++) {
+            const int pp = (tid >> 6) + 4 * m;
+            const uint32_t ip = ipc[m];
+            W[pp][qq] = (qv && pb + pp < n) ? co[0][ia][ip & 15] * co[1][ib][(ip >> 4) & 15] *
+                                                 co[2][ic][(ip >> 8) & 15] * co[3][id][(ip >> 12) & 15]
+                                           : 0.0;
          }
          __syncthreads();
-         if (jt * 16 < cols) {
-            double a[ECO_PB / 4];
+         if (pb + ECO_PB < n) {
 #pragma unroll
             for (int kk = 0; kk < ECO_PB / 4; kk++) {
-               const int p = pb + 4 * kk + (lane >> 4);
-               a[kk] = (jv && p < n) ? A0[(size_t)p * ncpt + src] : 0.0;
+               const int p = pb + ECO_PB + 4 * kk + (lane >> 4);
+               an[kk] = (jv && p < n) ? A0[(size_t)p * ncpt] : 0.0;
+               const int pp = pb + ECO_PB + (tid >> 6) + 4 * kk;
+               ipn[kk] = pp < n ? IP[pp] : 0u;
             }
-            if (f32) {
-               flt4 s32[4];
-#pragma unroll
-               for (int t = 0; t < 4; t++) s32[t] = flt4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-               for (int kk = 0; kk < ECO_PB / 4; kk++) {
-                  const double *Wk = &W[4 * kk + (lane >> 4)][lane & 15];
-#pragma unroll
-                  for (int t = 0; t < 4; t++)
-                     if (t < nqt) s32[t] = mfma_f32_16x16x4((float)a[kk], (float)Wk[16 * t], s32[t]);
-               }
-#pragma unroll
-               for (int t = 0; t < 4; t++)
-#pragma unroll
-                  for (int rr = 0; rr < 4; rr++) acc[t][rr] += (double)s32[t][rr];
-            } else {
-#pragma unroll
-               for (int kk = 0; kk < ECO_PB / 4; kk++) {
-                  const double *Wk = &W[4 * kk + (lane >> 4)][lane & 15];
-#pragma unroll
-                  for (int t = 0; t < 4; t++)
-                     if (t < nqt) acc[t] = mfma_f64_16x16x4(a[kk], Wk[16 * t], acc[t]);
-               }
+         }
+         if (tv) {
+            const double *Wl = &W[lane >> 4][lane & 15];
+            switch (nqt) {
+               case 1: eco_batch<1>(a, Wl, acc, f32); break;
+               case 2: eco_batch<2>(a, Wl, acc, f32); break;
+               case 3: eco_batch<3>(a, Wl, acc, f32); break;
+               default: eco_batch<4>(a, Wl, acc, f32); break;
             }
          }
       }
-      if (jt * 16 >= cols) continue;
+      if (!tv) continue;
       double *C = b.eco + (size_t)(r - b.r0) * e.row_doubles + e.eco_off[k];
 #pragma unroll
       for (int t = 0; t < 4; t++) {
-         const int q = q0 + 16 * t + (lane & 15);
-         if (t >= nqt || q >= nq) continue;
+         const int qo = q0 + 16 * t + (lane & 15);
+         if (t >= nqt || qo >= nq) continue;
 #pragma unroll
          for (int rr = 0; rr < 4; rr++) {
             // the two tiles place element rr of a lane in different rows
             const int jj = jt * 16 + (f32 ? ECO32_ROW(lane, rr) : 4 * rr + (lane >> 4));
-            if (jj < cols) C[(size_t)q * cols + jj] = acc[t][rr];
+            if (jj < cols) C[(size_t)qo * cols + jj] = acc[t][rr];
          }
       }
    }
@@ -204,7 +250,8 @@ int launch_eco(const TypePlan &tp, const BatchDev &b, const EcoArgs &e, void *st
    if (e.n_items <= 0 || b.r1 <= b.r0) return 0;
    (void)tp;
    const uint64_t nitems = (uint64_t)(b.r1 - b.r0) * (uint64_t)e.n_items;
-   const unsigned grid = (unsigned)std::min<uint64_t>(nitems, 65536);
+   unsigned grid = (unsigned)std::min<uint64_t>(nitems, 65536);
+   if (grid >= 64) grid &= ~7u;   // a multiple of the 8 XCDs (the kernel's item order)
    hipLaunchKernelGGL(k_eco, dim3(grid), dim3(256), 0, (hipStream_t)stream, b.tp, b, e);
    return hipGetLastError() == hipSuccess ? 0 : -1;
 }

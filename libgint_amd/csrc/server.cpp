@@ -151,6 +151,29 @@ class Server {
       return 0;
    }
 
+   void state(int64_t out[8])
+   {
+      std::unique_lock<std::mutex> lk(mu_);
+      int64_t run = 0, done = 0, refused = 0, res = 0, busy = 0;
+      for (const Request &r : reqs_) {
+         run += r.state == 1;
+         done += r.state == 2;
+         refused += r.state < 0;
+      }
+      for (const Client &c : clients_) {
+         res += c.resident;
+         busy += c.busy;
+      }
+      out[0] = (int64_t)queue_.size();
+      out[1] = run;
+      out[2] = done;
+      out[3] = refused;
+      out[4] = res;
+      out[5] = busy;
+      out[6] = (int64_t)used_;
+      out[7] = (int64_t)budget_;
+   }
+
    void stats(double out[8])
    {
       std::unique_lock<std::mutex> lk(mu_);
@@ -233,6 +256,14 @@ class Server {
       for (;;) {
          int64_t id;
          std::vector<int> evict;
+         // a copy of the request and pointers to the clients, taken under
+         // mu_: indexing a deque while submit / add_client push_back reads
+         // its block map, which growing reallocates (a worker could take a
+         // freed map, the wrong request or none); pointers to elements stay
+         // valid
+         Request req;
+         std::vector<Client *> ev;
+         Client *cp = nullptr;
          {
             std::unique_lock<std::mutex> lk(mu_);
             cv_.wait(lk, [&] {
@@ -244,9 +275,12 @@ class Server {
             if (id < 0) continue;
             reqs_[id].state = 1;
             reqs_[id].start = Clock::now();
+            req = reqs_[id];
+            cp = &clients_[req.client];
+            for (int i : evict) ev.push_back(&clients_[i]);
          }
-         for (int i : evict) {
-            Client &x = clients_[i];
+         for (Client *xp : ev) {
+            Client &x = *xp;
             x.e->plan_evict();
             if (x.dP) SCHECK(hipFree(x.dP));
             if (x.dK) SCHECK(hipFree(x.dK));
@@ -254,8 +288,8 @@ class Server {
             std::unique_lock<std::mutex> lk(mu_);
             x.busy = false;
          }
-         Request &r = reqs_[id];
-         Client &c = clients_[r.client];
+         const Request &r = req;
+         Client &c = *cp;
          if (c.e->plan_evicted()) c.e->plan_restore();
          if (!c.dP) {
             SCHECK(hipMalloc((void **)&c.dP, sizeof(double) * c.n));
@@ -270,8 +304,9 @@ class Server {
          const uint64_t measured = c.e->device_bytes() + 2 * 8 * c.n;
          {
             std::unique_lock<std::mutex> lk(mu_);
-            r.done = Clock::now();
-            r.state = 2;
+            Request &rd = reqs_[id];
+            rd.done = Clock::now();
+            rd.state = 2;
             // the footprint the budget counts from now on
             used_ = used_ - c.bytes + measured;
             c.bytes = measured;
@@ -323,6 +358,7 @@ void libgint_amd_server_set_budget(void *srv, int max_mem_MiB)
 {
    static_cast<gint::Server *>(srv)->set_budget((size_t)std::max(max_mem_MiB, 1) << 20);
 }
+void libgint_amd_server_state(void *srv, int64_t out[8]) { static_cast<gint::Server *>(srv)->state(out); }
 void libgint_amd_server_destroy(void *srv) { delete static_cast<gint::Server *>(srv); }
 
 }  // extern "C"

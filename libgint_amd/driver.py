@@ -428,11 +428,15 @@ class ServingClient:
         return self.P.copy()
 
 
-def serve(lib, clients, n_requests, rate, slots, max_mem_mib, seed=7, keep=False, resident_clients=None):
+def serve(lib, clients, n_requests, rate, slots, max_mem_mib, seed=7, keep=False, resident_clients=None,
+          progress=None, progress_s=10.0):
     """Poisson arrivals at `rate` requests/s, each to a client drawn at
     random, through one server; returns per-request (client, P, K or None,
     rc, latency, queueing, execution) and the server statistics.  With
-    resident_clients = n the budget is set to hold the n largest clients"""
+    resident_clients = n the budget is set to hold the n largest clients.
+    progress: a function of one string, called every progress_s seconds
+    with the server's state (queued, running, done, refused, resident and
+    busy clients, resident MiB) while requests are outstanding"""
     import threading
     import time
 
@@ -460,6 +464,19 @@ def serve(lib, clients, n_requests, rate, slots, max_mem_mib, seed=7, keep=False
 
     th = threading.Thread(target=waiter)
     th.start()
+    finished = threading.Event()
+
+    def watch():
+        state = (C.c_int64 * 8)()
+        while not finished.wait(progress_s):
+            lib.libgint_amd_server_state(srv, state)
+            progress(f"server: queued {state[0]} running {state[1]} done {state[2]} refused {state[3]}; "
+                     f"resident clients {state[4]} busy {state[5]}; resident {state[6] / 2**20:.0f} of "
+                     f"{state[7] / 2**20:.0f} MiB")
+
+    wt = threading.Thread(target=watch, daemon=True) if progress else None
+    if wt:
+        wt.start()
     t0 = time.perf_counter()
     t_next = 0.0
     for i in range(n_requests):
@@ -472,6 +489,9 @@ def serve(lib, clients, n_requests, rate, slots, max_mem_mib, seed=7, keep=False
         rid = lib.libgint_amd_server_submit(srv, ids[c], _ptr(P), _ptr(K))
         reqs[i] = (rid, c, P, K)
     th.join()
+    finished.set()
+    if wt:
+        wt.join()
     st = np.zeros(8)
     lib.libgint_amd_server_stats(srv, _ptr(st))
     lib.libgint_amd_server_destroy(srv)
