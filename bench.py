@@ -104,7 +104,7 @@ def serving(a):
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "p50_ms": round(st[1] * 1e3, 3), "p99_ms": round(st[2] * 1e3, 3),
         "config": {"workload": f"serving: {a.requests} K-build requests of {a.clients} periodic water-like systems "
-                               f"(20-200 atoms, 27 cells), Poisson arrivals at {a.rate} req/s, {a.slots} slots, "
+                               f"(20-200 atoms, 27 cells), Poisson arrivals at {a.rate} req/s, one worker, "
                                f"{a.budget} MiB device budget",
                    "config_index": 3, "requests": len(done), "served": len(ok), "wall_s": round(wall, 3),
                    "peak_resident_MiB": round(st[5] / 2**20, 1), "evictions": int(st[6]), "admissions": int(st[7]),
@@ -138,8 +138,9 @@ def main():
                     help="mixed-precision threshold tau for the extra mixed steps (0: none)")
     ap.add_argument("--clients", type=int, default=16, help="serving workload: systems")
     ap.add_argument("--requests", type=int, default=400, help="serving workload: requests")
-    ap.add_argument("--rate", type=float, default=20.0, help="serving workload: mean arrivals per second")
-    ap.add_argument("--slots", type=int, default=4, help="serving workload: concurrent requests")
+    ap.add_argument("--rate", type=float, default=5.0, help="serving workload: mean arrivals per second")
+    ap.add_argument("--slots", type=int, default=1,
+                    help="serving workload: worker threads asked of the server (it runs one)")
     ap.add_argument("--budget", type=int, default=7000, help="serving workload: device budget, MiB")
     ap.add_argument("--roof-only", action="store_true",
                     help="only the roofline's serialised sample run (for kernel-trace and counter profiles)")

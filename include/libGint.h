@@ -252,11 +252,12 @@ void libgint_amd_add_set_blocks(void *handle, double symm_fac, int ld_ac, int ld
    exchange matrix (P_size doubles each) on the device between requests.
    submit queues one K-build: P is read and K overwritten (host arrays of
    P_size doubles, kept valid until wait returns); requests run in arrival
-   order on `slots` worker threads as clients become idle and fit the device
+   order on a worker thread as clients become idle and fit the device
    budget (plan records, work buffers, tables, P, K of the resident clients
    <= max_mem_MiB; idle clients are evicted least recently used first, their
-   plans kept on the host), so requests that arrive while others run overlap
-   with them on the GPU.  wait blocks until a request is done: out[0] latency
+   plans kept on the host).  `slots` asks for several workers, whose
+   requests would overlap on the GPU; one runs (with four, long runs stopped
+   on the GPU, cause not found).  wait blocks until a request is done: out[0] latency
    (submit to K on the host), [1] queueing, [2] execution, seconds; returns 0,
    or -1 when the client can never fit the budget.  stats: out[0] requests
    done, [1] p50 and [2] p99 latency, [3] mean latency, [4] first arrival to
