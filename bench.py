@@ -122,6 +122,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--stride", type=int, default=1, help="every n-th set quartet of the workload (1: all)")
+    ap.add_argument("--sample", type=int, default=1,
+                    help="record a reproducible 1-in-N hash sample of the set quartets (1: all); the step time "
+                         "is reported for the sample and, scaled by N, as an estimate for the whole workload")
     ap.add_argument("--abi-stride", type=int, default=64,
                     help="per-call ABI ingestion is timed on every n-th set quartet and scaled up")
     ap.add_argument("--check-stride", type=int, default=0,
@@ -173,11 +176,12 @@ def main():
     c0 = read_c0(a.c0_file, 21) if a.c0_file else None
     t0 = time.time()
     s = D.System(a.config)
-    st = s.record()
+    st = s.record(sample=a.sample) if a.sample > 1 else s.record()
     t_rec = time.time() - t0
     n_sets = len(range(0, st.n_sets, a.stride))
     log(f"config {a.config}: {s.describe()}; sets {st.n_sets} rows {st.n_rows} primitive quartets {st.n_prm} "
-        f"contracted quartets {st.n_qrtt} (screened in {t_rec:.1f}s); stride {a.stride}: {n_sets} set quartets")
+        f"contracted quartets {st.n_qrtt} (screened in {t_rec:.1f}s); "
+        f"{'hash sample 1 in %d; ' % a.sample if a.sample > 1 else ''}stride {a.stride}: {n_sets} set quartets")
     P = s.P()
     dp = D._ptr
 
@@ -417,6 +421,9 @@ def main():
                    "contracted_quartets": st.n_qrtt if a.stride == 1 else None,
                    "surviving_slots_per_step": int(slots),
                    "kbuild_s_per_step": round(tot / a.steps, 4) if a.steps else None,
+                   "hash_sample": a.sample,
+                   "kbuild_s_per_step_whole_workload_est": (round(a.sample * tot / a.steps, 2)
+                                                            if a.steps and a.sample > 1 else None),
                    "abi_ingestion": abi, "set_level_ingestion_s": round(t_ing, 2),
                    "resident_plan_GiB": round(info[0] / 2**30, 3),
                    "path": "FP64, resident plan, fused Hermite cell sum per quartet type"},
