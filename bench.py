@@ -286,7 +286,7 @@ def main():
         lib.libgint_amd_tune_detail(h, tms, tpk)
         top = sorted(((tms[i], i) for i in range(256) if tms[i] > 0), reverse=True)[:12]
         log("autotune: classes by sampled time: " + ", ".join(
-            f"({i // 64}{(i // 16) % 4}|{(i // 4) % 4}{i % 4}) {ms:.1f}ms/{'twek'[tpk[i]]}" for ms, i in top))
+            f"({i // 64}{(i // 16) % 4}|{(i // 4) % 4}{i % 4}) {ms:.1f}ms/{'tweko'[tpk[i]]}" for ms, i in top))
         tune = {"classes_templated": int(tu[1]), "classes_workgroup": int(tu[2]), "classes_matrix_core": int(tu[3]),
                 "seconds": round(tu[4], 2)}
         lib.libgint_amd_destroy_handle(h)
@@ -335,7 +335,7 @@ def main():
     lib.libgint_amd_tune_detail(hr, tms, tpk)
     top = sorted(((tms[i], i) for i in range(256) if tms[i] > 0), reverse=True)[:16]
     log("roofline sample autotune: classes by sampled time: " + ", ".join(
-        f"({i // 64}{(i // 16) % 4}|{(i // 4) % 4}{i % 4}) {ms:.1f}ms/{'twek'[tpk[i]]}" for ms, i in top))
+        f"({i // 64}{(i // 16) % 4}|{(i // 4) % 4}{i % 4}) {ms:.1f}ms/{'tweko'[tpk[i]]}" for ms, i in top))
     for cat, nm in ((0, "fm_r by type L"), (1, "tq by type L"), (2, "gqs by type L"), (4, "eco by type L")):
         det = (C.c_double * 256)()
         lib.libgint_amd_timing_detail(hr, cat, det)

@@ -138,9 +138,10 @@ void libgint_amd_timing_detail(void *handle, int cat, double out[256]);
    durations without overlap, for profiling; results are unchanged); bit 2:
    wait for the work of each quartet type and report its completion on
    stderr (progress of long K-builds; results are unchanged); bit 3: resident
-   plans send every class that can take it to the matrix-core contraction
-   with the Kronecker HRR, whatever the autotuner chose (a test of that
-   path; results are unchanged) */
+   plans send every class that can take it to the Kronecker HRR, whatever
+   the autotuner chose, with the matrix-core contraction where the class
+   has one (else its own); bit 4: the same with the class's own contraction
+   where it fits (tests of that path; results are unchanged) */
 void libgint_amd_set_debug(void *handle, int flags);
 /* libgint_amd_plan_execute replays the plan's launches (every chunk's
    kernels on the handle's stream and its side streams) as one HIP graph:
@@ -176,13 +177,14 @@ void libgint_amd_set_mixed_precision(void *handle, double tau);
    by the last execution, [3] set quartets of the plan */
 void libgint_amd_mixed_info(void *handle, double out[4]);
 /* out[0] the plan is tuned, [1..3] classes given the templated kernel /
-   the workgroup kernel / the matrix-core contraction (with either HRR),
-   [4] seconds spent */
+   the workgroup kernel with its own contraction / the matrix-core
+   contraction (each with either HRR), [4] seconds spent */
 void libgint_amd_tune_info(void *handle, double out[5]);
 /* per class code ((la*4+lb)*4+lc)*4+ld: ms[] the chosen schedule's time on
    the timing samples (summed over quartet types), pick[] that schedule
    (0 templated, 1 workgroup, 2 matrix-core contraction + layered HRR,
-   3 matrix-core contraction + Kronecker HRR; last type timed) */
+   3 matrix-core contraction + Kronecker HRR, 4 workgroup kernel's own
+   contraction + Kronecker HRR; last type timed) */
 void libgint_amd_tune_detail(void *handle, double ms[256], int pick[256]);
 /* resident plans: the calls between plan_begin and plan_end are kept on the
    device; plan_execute adds their exchange terms to K for the density of the

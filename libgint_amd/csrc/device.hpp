@@ -328,7 +328,9 @@ struct CqLayout {
    int o_kron;      // LDS of the Kronecker path: bra components x ket spherical pairs (whole 16x16 tiles), T tables
    int kron_ws;     // doubles of the Kronecker path's row maps (up to 8 rows); 0: the class cannot take the path
    int o_kmap;      // LDS offset of those maps, or -1: in the workgroup's HBM slice
-   int kron;        // the class takes the Kronecker path (with eco; a schedule of the autotuner)
+   int kron;        // the class takes the Kronecker path (a schedule of the autotuner)
+   int o_kc;        // LDS of one contraction's blocks and CQ_PCH coefficient products for the Kronecker path
+                    // with the class's own contraction (no eco), or -1: that schedule does not fit
 };
 
 // address of the no-allocation scratch array on the current device, looked

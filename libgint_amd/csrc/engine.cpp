@@ -327,7 +327,7 @@ int Engine::type_of(int i, int j, int k, int l)
          ti.cplans.push_back(make_plan(code / 64, (code / 16) % 4, (code / 4) % 4, code % 4));
       }
       make_cq_plan(ti.plan, ti.cplans, ti.lays, ti.srcmaps, &ti.cq_lds, &ti.cq_gws);
-      for (const CqLayout &y : ti.lays) ti.cq_wide = ti.cq_wide || (y.kron && y.eco && !y.skip);
+      for (const CqLayout &y : ti.lays) ti.cq_wide = ti.cq_wide || (y.kron && !y.skip);
       make_eco_items(ti);
       types_.push_back(ti);
       q_.emplace_back();
@@ -819,7 +819,7 @@ void Engine::plan_execute()
    if (!prepare_exec(plan_dev_, plan_max_index_, w, kt, FP_size_)) return;
    // (not while an integral-output array is set: its classes write there)
    if (autotune_on_ && !plan_tuned_ && !d_out_) autotune(w, kt);
-   if (debug_ & 8) force_kron();
+   if (debug_ & 24) force_kron((debug_ & 16) != 0);
    mixed_mode_ = 0;
    if (mixed_tau_ > 0.0 && !d_out_) {
       ensure_mixed();
@@ -1573,25 +1573,32 @@ void Engine::apply_schedule(TypeInfo &ti, int k, int v)
 {
    CqLayout &y = ti.lays[k];
    y.skip = v == 0 ? 1 : 0;
-   y.eco = v >= 2 ? 1 : 0;
-   y.kron = v == 3 ? 1 : 0;
+   y.eco = (v == 2 || v == 3) ? 1 : 0;
+   y.kron = v >= 3 ? 1 : 0;
 }
 
-// debug bit 3: every class of the plan that can take the Kronecker path
-// (with the matrix-core contraction) takes it, whatever the schedule
-void Engine::force_kron()
+// debug bits 3 and 4: every class of the plan that can take the Kronecker
+// path takes it, whatever the schedule: bit 3 with the matrix-core
+// contraction where the class has a place in the contraction buffer (else
+// its own), bit 4 with its own contraction where that fits (else the matrix
+// cores)
+void Engine::force_kron(bool own_first)
 {
    for (auto &D : plan_dev_) {
       if (D.chunks.empty()) continue;
       TypeInfo &ti = types_[D.type];
       bool any = false;
-      for (CqLayout &y : ti.lays)
-         if (y.eco_off >= 0 && y.kron_ws && !(y.kron && y.eco && !y.skip)) {
-            y.skip = 0;
-            y.eco = 1;
-            y.kron = 1;
-            any = true;
-         }
+      for (CqLayout &y : ti.lays) {
+         if (!y.kron_ws) continue;
+         const bool can_eco = y.eco_off >= 0, can_own = y.o_kc >= 0;
+         if (!can_eco && !can_own) continue;
+         const int eco = (can_eco && (!own_first || !can_own)) ? 1 : 0;
+         if (y.kron && !y.skip && y.eco == eco) continue;
+         y.skip = 0;
+         y.eco = eco;
+         y.kron = 1;
+         any = true;
+      }
       if (any) upload_layouts(ti);
    }
 }
@@ -1599,7 +1606,7 @@ void Engine::force_kron()
 void Engine::upload_layouts(TypeInfo &ti)
 {
    ti.cq_wide = false;
-   for (const CqLayout &y : ti.lays) ti.cq_wide = ti.cq_wide || (y.kron && y.eco && !y.skip);
+   for (const CqLayout &y : ti.lays) ti.cq_wide = ti.cq_wide || (y.kron && !y.skip);
    assign_parts(ti.plan, ti.cplans, ti.lays);
    make_eco_items(ti);
    HIPCHECK(hipMemcpy(ti.d_lays, ti.lays.data(), sizeof(CqLayout) * ti.lays.size(), hipMemcpyHostToDevice));
@@ -1698,27 +1705,31 @@ void Engine::autotune(const RunBufs &w, const KTargets &kt)
          for (int k = 0; k < tp.nclass; k++) {
             if (!D.class_used[k] || dec[k] >= 0) continue;
             double best = 1e300;
-            int bv = keep[k].skip ? 0 : (keep[k].eco ? (keep[k].kron ? 3 : 2) : 1);
+            int bv = keep[k].skip ? 0 : (keep[k].eco ? (keep[k].kron ? 3 : 2) : (keep[k].kron ? 4 : 1));
             // the sample's contraction FLOPs of the class: above TUNE_VALU_FLOPS
             // the VALU contraction loop (schedule 1) is not timed where the
             // matrix cores can take the class (it is slower there by an order
             // of magnitude or two, and timing it would cost seconds)
             const double cflops = 2.0 * tp.cls_nl[k] * (double)tp.cls_ncp[k] * (double)(c.p1 - c.p0);
-            for (int v = 0; v < 4; v++) {
+            for (int v = 0; v < 5; v++) {
                if (v == 0 && !tq_has(tp.cls_code[k])) continue;
-               if (v >= 2 && keep[k].eco_off < 0) continue;
-               if (v == 3 && !keep[k].kron_ws) continue;
+               if ((v == 2 || v == 3) && keep[k].eco_off < 0) continue;
+               if (v >= 3 && !keep[k].kron_ws) continue;
+               if (v == 4 && keep[k].o_kc < 0) continue;
+               // the own contraction is not timed where the VALU
+               // contraction would be too slow (as schedule 1)
+               if (v == 4 && keep[k].eco_off >= 0 && cflops > TUNE_VALU_FLOPS) continue;
                if (v == 1 && keep[k].eco_off >= 0 && cflops > TUNE_VALU_FLOPS) continue;
                // this class alone in part 0 with schedule v
                std::vector<CqLayout> L = keep;
                for (auto &y : L) y.part = -1;
                L[k].skip = v == 0 ? 1 : 0;
-               L[k].eco = v >= 2 ? 1 : 0;
-               L[k].kron = v == 3 ? 1 : 0;
+               L[k].eco = (v == 2 || v == 3) ? 1 : 0;
+               L[k].kron = v >= 3 ? 1 : 0;
                L[k].part = 0;
                HIPCHECK(hipMemcpy(ti.d_lays, L.data(), sizeof(CqLayout) * L.size(), hipMemcpyHostToDevice));
                std::vector<EcoItem> items;
-               if (v >= 2) {
+               if (v == 2 || v == 3) {
                   for (int jt0 = 0; jt0 * 16 < tp.cls_ncp[k]; jt0 += 4)
                      for (int qb = 0; qb * 64 < tp.cls_nl[k]; qb++)
                         items.push_back(EcoItem{(uint16_t)k, (uint16_t)jt0, (uint16_t)qb, 0});
@@ -1738,7 +1749,7 @@ void Engine::autotune(const RunBufs &w, const KTargets &kt)
                   if (v == 0) {
                      rc = launch_tq(k, tp, t, b, ks, ti.d_srcmaps + keep[k].o_src, st);
                   } else {
-                     if (v >= 2) {
+                     if (v == 2 || v == 3) {
                         EcoArgs e{};
                         e.items = d_items;
                         e.n_items = (int)items.size();
@@ -1750,7 +1761,7 @@ void Engine::autotune(const RunBufs &w, const KTargets &kt)
                      }
                      if (!rc)
                         rc = launch_gqs(tp, ti.d_cps, ti.d_lays, ti.d_srcmaps, ti.cq_lds, ti.cq_gws, 0, 1, t, b, ks,
-                                        w.gws, st, v == 3 || ti.cq_wide);
+                                        w.gws, st, v >= 3 || ti.cq_wide);
                   }
                   if (rc) fatal("autotune: candidate launch", rc);
                   HIPCHECK(hipEventRecord(e1, st));

@@ -169,7 +169,7 @@ class Engine {
    void timing_detail(int cat, double out[256]);
    void reset_counters();
    void set_debug(int flags) { debug_ = flags; }
-   void force_kron();
+   void force_kron(bool own_first);
    void set_graph(bool on) { graph_on_ = on; }
    void set_autotune(bool on) { autotune_on_ = on; }
    void set_mixed_precision(double tau) { mixed_tau_ = tau > 0.0 ? tau : 0.0; }
@@ -179,6 +179,7 @@ class Engine {
       out[0] = plan_tuned_ ? 1.0 : 0.0;
       for (int i = 0; i < 3; i++) out[1 + i] = (double)tune_counts_[i];
       out[3] += (double)tune_counts_[3];
+      out[2] += (double)tune_counts_[4];
       out[4] = tune_seconds_;
    }
    // per class code: ms of the chosen schedule on the timing samples, and the schedule
@@ -333,7 +334,8 @@ class Engine {
    static constexpr double TUNE_VALU_FLOPS = 2e8;   // largest sample contraction timed on the VALU
    bool autotune_on_ = true;
    bool plan_tuned_ = false;
-   uint64_t tune_counts_[4] = {0, 0, 0, 0};       // classes given k_tqw / k_gqs / k_eco + k_gqs / k_eco + Kronecker k_gqs
+   // classes given k_tqw / k_gqs / k_eco + k_gqs / k_eco + Kronecker k_gqs / Kronecker k_gqs with its own contraction
+   uint64_t tune_counts_[5] = {0, 0, 0, 0, 0};
    double tune_seconds_ = 0.0;
    double tune_ms_[256] = {};
    int tune_pick_[256] = {};

@@ -1246,7 +1246,8 @@ __device__ __forceinline__ double kron_m(int l1, int l2, int s, int g, const dou
 // integrals) as cq_one does.
 template <int NT>
 __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b,
-                        const KTargets &kt, const CqLayout &y, double *W, uint32_t gi, const char *tabs)
+                        const KTargets &kt, const CqLayout &y, double *W, uint32_t gi, const char *tabs,
+                        const double *co, const int *__restrict__ srcmap)
 {
    static_assert(NT == 256, "one 16-row band of the integrals per wavefront, four of them");
    extern __shared__ double lds[];
@@ -1316,7 +1317,34 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
          if (!cached) build(r, 0);
          __syncthreads();   // the maps are written, the previous Y is no longer read
          const double *Mk = W + (size_t)(cached ? r - g.row0 : 0) * MS, *Mb = Mk + nscd * nket;
-         const double *Cl = b.eco + (size_t)(r - b.r0) * b.eco_row + y.eco_off + (size_t)l * ncp;
+         const double *Cl;
+         if (y.eco) {
+            // contracted on the matrix cores (k_eco)
+            Cl = b.eco + (size_t)(r - b.r0) * b.eco_row + y.eco_off + (size_t)l * ncp;
+         } else {
+            // the class's own contraction of the row's primitive blocks for
+            // contraction l, into LDS (as the layered path's, one contraction)
+            double *Cs = lds + y.o_kc, *Kp = Cs + ncp;
+            const RowRec row = b.rows[r];
+            const uint32_t *IP = b.prm_ip + row.prm_off;
+            const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * tp.ncp;
+            for (int j = tid; j < ncp; j += NT) Cs[j] = 0.0;
+            for (int pc = 0; pc < (int)row.n_prm; pc += CQ_PCH) {
+               const int np = min(CQ_PCH, (int)row.n_prm - pc);
+               __syncthreads();   // the previous step's products are no longer read
+               if (tid < CQ_PCH) Kp[tid] = tid < np ? kprod_lds(tp, co, cls_k, l, IP[pc + tid]) : 0.0;
+               __syncthreads();
+               const double *A = A0 + (size_t)pc * tp.ncp;
+               for (int j = tid; j < ncp; j += NT) {
+                  const int src = srcmap[j];
+                  double sum = 0.0;
+                  for (int pp = 0; pp < np; pp++) sum += Kp[pp] * A[(size_t)pp * tp.ncp + src];
+                  Cs[j] += sum;
+               }
+            }
+            __syncthreads();
+            Cl = Cs;
+         }
          // Y = C M_ket^T, wavefront wv the tiles wv, wv + 4, ... (one
          // accumulator tile at a time: the integrals' four hold registers)
          for (int tile = wv; tile < TM1 * TN; tile += 4) {
@@ -1481,9 +1509,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       // the Kronecker path where it applies: contraction on the matrix
       // cores, layers beyond LDS, and no class-dependent screening to add
       bool kron = false;
-      if constexpr (NT == 256) kron = y.kron && y.eco && y.kron_ws && min(b.fix_count[0], b.fix_cap) == 0;
+      if constexpr (NT == 256)
+         kron = y.kron && y.kron_ws && (y.eco || y.o_kc >= 0) && min(b.fix_count[0], b.fix_cap) == 0;
       if (kron)
-         cq_kron<256>(*cpl, k, tp, t, b, kt, y, y.o_kmap >= 0 ? lds_base() + y.o_kmap : WS, gi, tabs);
+         cq_kron<256>(*cpl, k, tp, t, b, kt, y, y.o_kmap >= 0 ? lds_base() + y.o_kmap : WS, gi, tabs, co,
+                      srcmaps + y.o_src);
       else
          cq_one<NT>(*cpl, k, tp, t, b, kt, y, WS, srcmaps + y.o_src, gi, co, ipb, tabs);
    }
@@ -1565,9 +1595,14 @@ static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget)
    const int kron_lds = 16 * std::max((nbra + 15) / 16, (nsab + 15) / 16) * 16 * ((nscd + 15) / 16) +
                         (2 * cp.ld + 1) * ioff3(cp.ld + 1) + (2 * cp.lb + 1) * ioff3(cp.lb + 1);
    const int maps = nscd * nket + nsab * nbra;
+   // with the class's own contraction (no eco): one contraction's blocks
+   // and the coefficient products of CQ_PCH primitive quartets, after the
+   // block and T tables
+   const int kcneed = cp.ncp + CQ_PCH;
    y.o_kron = 0;
    y.kron_ws = 0;
    y.o_kmap = -1;
+   y.o_kc = -1;
    y.kron = 0;
    int kbytes = 0;
    if (y.ws_global) {
@@ -1576,12 +1611,21 @@ static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget)
       if (kron_lds <= budget) {
          y.kron_ws = std::max(1, std::min(8, y.lws / maps)) * maps;
          kbytes = kron_lds;
+         if (kron_lds + kcneed <= budget) {
+            y.o_kc = kron_lds;
+            kbytes += kcneed;
+         }
       }
-   } else if (kron_lds + maps <= budget) {
+   } else {
       // the maps of up to 8 rows in LDS, within the type's budget
-      y.o_kmap = kron_lds;
-      y.kron_ws = std::min(8, (budget - kron_lds) / maps) * maps;
-      kbytes = kron_lds + y.kron_ws;
+      const bool own = kron_lds + kcneed + maps <= budget;
+      const int base = kron_lds + (own ? kcneed : 0);
+      if (base + maps <= budget) {
+         y.o_kc = own ? kron_lds : -1;
+         y.o_kmap = base;
+         y.kron_ws = std::min(8, (budget - base) / maps) * maps;
+         kbytes = base + y.kron_ws;
+      }
    }
    y.bytes = 8 * std::max(off, kbytes);
    return y;

@@ -1,9 +1,12 @@
 """The Kronecker HRR of the per-class kernel (the HRR of each side unrolled
 into a dense map per row, the spherical transform folded in, two FP64
 matrix-core products per contraction and row), forced for every class that
-can take it (debug bit 3: the classes with many contractions of an spdf
-centre, with their maps in LDS or in HBM), against the oracle and against
-the same plan on the schedules the autotuner chose, to the parity bar."""
+can take it (debug bit 3: with the matrix-core contraction where the class
+has a place in the contraction buffer, the classes with many contractions
+of an spdf centre; bit 4: with the class's own contraction where it fits,
+every class of small and mid size; maps in LDS or in HBM), against the
+oracle and against the same plan on the schedules the autotuner chose, to
+the parity bar."""
 import numpy as np
 import pytest
 
@@ -26,7 +29,7 @@ def test_kron_path_against_oracle(n_tm, seed):
     o.oracle_destroy_handle(ho)
     kmax = np.max(np.abs(Ko))
     Ks = []
-    for debug in (0, 8):
+    for debug in (0, 8, 16):
         h, table = D.setup_handle(lib, "libgint_", s, P=P)
         lib.libgint_amd_plan_begin(h)
         st.replay(table, h)
@@ -42,4 +45,5 @@ def test_kron_path_against_oracle(n_tm, seed):
             assert err <= KTOL_ABS + KTOL_REL * kmax, (debug, it, err)
         Ks.append(K)
         lib.libgint_amd_destroy_handle(h)
-    assert np.max(np.abs(Ks[0] - Ks[1])) <= KTOL_ABS + KTOL_REL * kmax
+    for Kf in Ks[1:]:
+        assert np.max(np.abs(Ks[0] - Kf)) <= KTOL_ABS + KTOL_REL * kmax
