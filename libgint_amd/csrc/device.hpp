@@ -331,6 +331,10 @@ struct CqLayout {
    int kron;        // the class takes the Kronecker path (a schedule of the autotuner)
    int o_kc;        // LDS of one contraction's blocks and CQ_PCH coefficient products for the Kronecker path
                     // with the class's own contraction (no eco), or -1: that schedule does not fit
+   int o_kacc;      // LDS of the K accumulators (the four K blocks of a set quartet, summed over its
+                    // contractions before the atomics), or -1
+   int kacc;        // doubles of the accumulators per spin
+   int kacc_cap;    // doubles reserved for them (kacc or 2 kacc: one or two spins)
 };
 
 // address of the no-allocation scratch array on the current device, looked

@@ -820,6 +820,95 @@ __device__ __forceinline__ unsigned blk_idx(bool T, unsigned ld, unsigned row_i,
    return T ? row_i * ld + col_i : col_i * ld + row_i;
 }
 
+// symmetry factor of a K layout code: 2, 1, 1/2, 1/4, or 0 (code 4)
+__device__ __forceinline__ double symm_code_fac(uint32_t c) { return c >= 4 ? 0.0 : (c == 0 ? 2.0 : (c == 1 ? 1.0 : (c == 2 ? 0.5 : 0.25))); }
+
+// The K updates of one contraction (ia, ib, ic, id), summed into the set
+// quartet's K accumulators in LDS instead of four atomics per integral.
+// Per spin the accumulators are the four K blocks, [ac: NA x NC][ad: NA x ND]
+// [bc: NB x NC][bd: NB x ND], rows and columns (contraction, spherical
+// function) of their centres.  Each element is the sum, over the two other
+// spherical indices, of the integrals times the P block it pairs with (the
+// reference's updates K_bd += I P_ac, K_bc += I P_ad, K_ad += I P_bc,
+// K_ac += I P_bd per integral, src/compute_KS.cpp:8-117, regrouped; the
+// factor -f is applied at the flush).  Thread t takes the same elements for
+// every contraction, so the contractions a thread adds one after the other
+// need no synchronisation.
+template <int NT, typename IntegralFn>
+__device__ __forceinline__ void kacc_add(const KTargets &kt, double *acc, int kacc, const size_t off[4], uint32_t flags,
+                                         uint32_t ld01, uint32_t ld23, int ia, int ib, int ic, int id, int nsa, int nsb,
+                                         int nsc, int nsd, int NA, int NB, int NC, int ND, IntegralFn I)
+{
+   const unsigned ldac = ld01 & 0xffff, ldad = ld01 >> 16, ldbc = ld23 & 0xffff, ldbd = ld23 >> 16;
+   const bool Tac = flags & 1, Tad = flags & 2, Tbc = flags & 4, Tbd = flags & 8;
+   const int n_bd = nsb * nsd, n_bc = nsb * nsc, n_ad = nsa * nsd, n_ac = nsa * nsc;
+   const int ntask = n_bd + n_bc + n_ad + n_ac;
+   for (int t = threadIdx.x; t < ntask; t += NT) {
+      for (int sp = 0; sp < kt.nspin; sp++) {
+         const double *P = kt.P[sp];
+         double *A = acc + sp * kacc;
+         double sum = 0.0;
+         if (t < n_bd) {
+            const int isb = t / nsd, isd = t - isb * nsd;
+            for (int isa = 0; isa < nsa; isa++)
+               for (int isc = 0; isc < nsc; isc++) sum += I(isa, isb, isc, isd) * P[off[0] + blk_idx(Tac, ldac, isa, isc)];
+            A[NA * NC + NA * ND + NB * NC + (ib * nsb + isb) * ND + id * nsd + isd] += sum;
+         } else if (t < n_bd + n_bc) {
+            const int u = t - n_bd, isb = u / nsc, isc = u - isb * nsc;
+            for (int isa = 0; isa < nsa; isa++)
+               for (int isd = 0; isd < nsd; isd++) sum += I(isa, isb, isc, isd) * P[off[1] + blk_idx(Tad, ldad, isa, isd)];
+            A[NA * NC + NA * ND + (ib * nsb + isb) * NC + ic * nsc + isc] += sum;
+         } else if (t < n_bd + n_bc + n_ad) {
+            const int u = t - n_bd - n_bc, isa = u / nsd, isd = u - isa * nsd;
+            for (int isb = 0; isb < nsb; isb++)
+               for (int isc = 0; isc < nsc; isc++) sum += I(isa, isb, isc, isd) * P[off[2] + blk_idx(Tbc, ldbc, isb, isc)];
+            A[NA * NC + (ia * nsa + isa) * ND + id * nsd + isd] += sum;
+         } else {
+            const int u = t - n_bd - n_bc - n_ad, isa = u / nsc, isc = u - isa * nsc;
+            for (int isb = 0; isb < nsb; isb++)
+               for (int isd = 0; isd < nsd; isd++) sum += I(isa, isb, isc, isd) * P[off[3] + blk_idx(Tbd, ldbd, isb, isd)];
+            A[(ia * nsa + isa) * NC + ic * nsc + isc] += sum;
+         }
+      }
+   }
+}
+
+// the K accumulators into K: one atomic per element of the four blocks
+template <int NT>
+__device__ void kacc_flush(const KTargets &kt, const KsGroup &ks, const double *acc, int kacc, int nsa, int nsb, int nsc,
+                           int nsd, int NA, int NB, int NC, int ND)
+{
+   const double fac = symm_code_fac((ks.flags >> 4) & 7) * kt.hf_fac;
+   if (fac == 0.0) return;
+   const unsigned ldac = ks.ld01 & 0xffff, ldad = ks.ld01 >> 16, ldbc = ks.ld23 & 0xffff, ldbd = ks.ld23 >> 16;
+   const bool Tac = ks.flags & 1, Tad = ks.flags & 2, Tbc = ks.flags & 4, Tbd = ks.flags & 8;
+   for (int sp = 0; sp < kt.nspin; sp++) {
+      double *K = kt.K[sp];
+      for (int e = threadIdx.x; e < kacc; e += NT) {
+         const double v = acc[sp * kacc + e];
+         if (v == 0.0) continue;
+         long long o;
+         if (e < NA * NC) {
+            const int r = e / NC, c = e - r * NC, i = r / nsa, si = r - i * nsa, j = c / nsc, sj = c - j * nsc;
+            o = (long long)ks.base[0] + (long long)i * ks.step[0] + (long long)j * ks.step[1] + blk_idx(Tac, ldac, si, sj);
+         } else if (e < NA * NC + NA * ND) {
+            const int u = e - NA * NC, r = u / ND, c = u - r * ND, i = r / nsa, si = r - i * nsa, j = c / nsd,
+                      sj = c - j * nsd;
+            o = (long long)ks.base[1] + (long long)i * ks.step[2] + (long long)j * ks.step[3] + blk_idx(Tad, ldad, si, sj);
+         } else if (e < NA * NC + NA * ND + NB * NC) {
+            const int u = e - NA * NC - NA * ND, r = u / NC, c = u - r * NC, i = r / nsb, si = r - i * nsb, j = c / nsc,
+                      sj = c - j * nsc;
+            o = (long long)ks.base[2] + (long long)i * ks.step[4] + (long long)j * ks.step[5] + blk_idx(Tbc, ldbc, si, sj);
+         } else {
+            const int u = e - NA * NC - NA * ND - NB * NC, r = u / ND, c = u - r * ND, i = r / nsb, si = r - i * nsb,
+                      j = c / nsd, sj = c - j * nsd;
+            o = (long long)ks.base[3] + (long long)i * ks.step[6] + (long long)j * ks.step[7] + blk_idx(Tbd, ldbd, si, sj);
+         }
+         unsafeAtomicAdd(&K[(size_t)o], -fac * v);
+      }
+   }
+}
+
 // one (set quartet, class): the work of k_gqs for class cls_k of group gi
 template <int NT>
 __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b,
@@ -853,6 +942,12 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
    const bool expl = (ks.flags & KS_EXPLICIT) != 0;
    const bool out_mode = (ks.flags & KS_OUTPUT) != 0;
    const bool reduce = y.reduce && !expl && !out_mode && nq == nq_all;
+   // K accumulators in LDS when the integrals of all contractions do not fit
+   const bool use_acc = !reduce && y.o_kacc >= 0 && !expl && !out_mode && nq == nq_all && kt.nspin * y.kacc <= y.kacc_cap;
+   const int NA = nla * nsa, NB = nlb * nsb, NC = nlc * nsc, ND = nld * nsd;
+   double *acc = lds + y.o_kacc;
+   if (use_acc)
+      for (int i = tid; i < kt.nspin * y.kacc; i += NT) acc[i] = 0.0;
    for (int l0 = 0; l0 < nq; l0 += y.lb) {
       const int nb = min(y.lb, nq - l0);
       __syncthreads();
@@ -1047,6 +1142,21 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
          }
          continue;
       }
+      if (use_acc) {
+         // the batch's contractions into the K accumulators, one after the other
+         for (int l = 0; l < nb; l++) {
+            const int q = l0 + l;
+            const int ia = q / (nlb * nlc * nld), ib = (q / (nlc * nld)) % nlb, ic = (q / nld) % nlc, id = q % nld;
+            size_t off[4];
+            ks_offsets(ks, nlb, nlc, nld, q, off);
+            const double *Il = WS + l * lws + cap;
+            kacc_add<NT>(kt, acc, y.kacc, off, ks.flags, ks.ld01, ks.ld23, ia, ib, ic, id, nsa, nsb, nsc, nsd, NA, NB,
+                         NC, ND, [&](int isa, int isb, int isc, int isd) {
+                            return Il[((isa * nsb + isb) * nsc + isc) * nsd + isd];
+                         });
+         }
+         continue;
+      }
       // one contraction at a time: 4 atomic updates per spherical integral
       for (int idx = tid; idx < nb * Ns; idx += NT) {
          const int l = fdiv(idx, Ns), tt = idx - l * Ns;
@@ -1079,6 +1189,11 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
          }
       }
    }
+   if (use_acc) {
+      __syncthreads();
+      kacc_flush<NT>(kt, ks, acc, y.kacc, nsa, nsb, nsc, nsd, NA, NB, NC, ND);
+      return;
+   }
    if (!reduce) return;
    __syncthreads();
    // the same updates, summed over the set quartet's contractions first:
@@ -1087,7 +1202,6 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
    if (fac == 0.0) return;
    const unsigned ldac = ks.ld01 & 0xffff, ldad = ks.ld01 >> 16, ldbc = ks.ld23 & 0xffff, ldbd = ks.ld23 >> 16;
    const bool Tac = ks.flags & 1, Tad = ks.flags & 2, Tbc = ks.flags & 4, Tbd = ks.flags & 8;
-   const int NA = nla * nsa, NB = nlb * nsb, NC = nlc * nsc, ND = nld * nsd;
    auto Iat = [&](int ia, int isa, int ib, int isb, int ic, int isc, int id, int isd) {
       return Iall[(((ia * nlb + ib) * nlc + ic) * nld + id) * Ns + ((isa * nsb + isb) * nsc + isc) * nsd + isd];
    };
@@ -1281,6 +1395,14 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
    const bool out_mode = (ks.flags & KS_OUTPUT) != 0;
    const uint32_t nr = g.row1 - g.row0;
    const bool cached = nr <= (uint32_t)rc;
+   // K accumulators in LDS (summed over the contractions before the atomics)
+   const int nla = tp.nl[0][sh & 15];
+   const bool use_acc = y.o_kacc >= 0 && !expl && !out_mode && nq == nla * nlb * nlc * nld &&
+                        kt.nspin * y.kacc <= y.kacc_cap;
+   const int NA = nla * nsa, NB = nlb * nsb, NC = nlc * nsc, ND = nld * nsd;
+   double *kap = lds + y.o_kacc;
+   if (use_acc)
+      for (int i = tid; i < kt.nspin * y.kacc; i += NT) kap[i] = 0.0;
 
    // the two maps of row r into slot m of the slice
    auto build = [&](uint32_t r, int m) {
@@ -1401,6 +1523,16 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
                for (int rr = 0; rr < 4; rr++)
                   Y[(16 * wv + 4 * rr + (lane >> 4)) * YS + 16 * tn + (lane & 15)] = acc[tn][rr];
       __syncthreads();
+      if (use_acc) {
+         const int ia = l / (nlb * nlc * nld), ib = (l / (nlc * nld)) % nlb, ic = (l / nld) % nlc, id = l % nld;
+         size_t off[4];
+         ks_offsets(ks, nlb, nlc, nld, l, off);
+         kacc_add<NT>(kt, kap, y.kacc, off, ks.flags, ks.ld01, ks.ld23, ia, ib, ic, id, nsa, nsb, nsc, nsd, NA, NB, NC,
+                      ND, [&](int isa, int isb, int isc, int isd) {
+                         return Y[(isa * nsb + isb) * YS + isc * nsd + isd];
+                      });
+         continue;
+      }
       // its K updates (or its integrals, in the TRA order)
       size_t off[4] = {0, 0, 0, 0};
       uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
@@ -1438,6 +1570,10 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
             unsafeAtomicAdd(&K[Iac], iabcd * P[Ibd]);
          }
       }
+   }
+   if (use_acc) {
+      __syncthreads();
+      kacc_flush<NT>(kt, ks, kap, y.kacc, nsa, nsb, nsc, nsd, NA, NB, NC, ND);
    }
 }
 
@@ -1561,12 +1697,20 @@ int launch_fixup(const TypePlan &tp, const DevTables &t, const BatchDev &b, uint
 
 // LDS plan of one class in k_gqs: as many contractions per pass as fit in
 // `budget` doubles, the workspace in HBM when not even one does
-static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget)
+static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget_all, int kacc)
 {
    CqLayout y;
    y.lws = 3 * cp.ycap + cp.Nc;
    const int need_I = nq * cp.Ns;
    y.reduce = need_I <= 2048 ? 1 : 0;
+   // K accumulators where the integrals of all contractions do not fit: the
+   // four K blocks of the set quartet (kacc doubles per spin), within half
+   // the budget, for two spins or else one
+   int kcap = 0;
+   if (!y.reduce && kacc > 0) kcap = 2 * kacc <= budget_all / 2 ? 2 * kacc : (kacc <= budget_all / 2 ? kacc : 0);
+   const int budget = budget_all - kcap;
+   y.kacc = kcap ? kacc : 0;
+   y.kacc_cap = kcap;
    const int fixed = (y.reduce ? need_I : 0) + CQ_PCH;
    y.ws_global = 0;
    int lb = (budget - fixed) / (y.lws + CQ_PCH);
@@ -1627,7 +1771,8 @@ static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget)
          kbytes = base + y.kron_ws;
       }
    }
-   y.bytes = 8 * std::max(off, kbytes);
+   y.o_kacc = kcap ? std::max(off, kbytes) : -1;
+   y.bytes = 8 * (std::max(off, kbytes) + kcap);
    return y;
 }
 
@@ -1648,7 +1793,10 @@ void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::ve
    int eco_row = 0;
    for (int k = 0; k < tp.nclass; k++) {
       const ClassPlan &cp = cps[k];
-      CqLayout y = cq_layout(cp, tp.cls_nl[k], budget);
+      const int sh = tp.cls_sh[k];
+      const int NA = tp.nl[0][sh & 15] * (2 * cp.la + 1), NB = tp.nl[1][(sh >> 4) & 15] * (2 * cp.lb + 1),
+                NC = tp.nl[2][(sh >> 8) & 15] * (2 * cp.lc + 1), ND = tp.nl[3][(sh >> 12) & 15] * (2 * cp.ld + 1);
+      CqLayout y = cq_layout(cp, tp.cls_nl[k], budget, NA * NC + NA * ND + NB * NC + NB * ND);
       y.skip = tq_has(tp.cls_code[k]) ? 1 : 0;
       // many contractions: contracted on the matrix cores (k_eco), a row's
       // contraction buffer holding the class's nq x ncp doubles.  Every
