@@ -329,6 +329,7 @@ struct CqLayout {
    int kron_ws;     // doubles of the Kronecker path's row maps (up to 8 rows); 0: the class cannot take the path
    int o_kmap;      // LDS offset of those maps, or -1: in the workgroup's HBM slice
    int kron;        // the class takes the Kronecker path (a schedule of the autotuner)
+   int kron_kb;     // its contractions per pass
    int o_kc;        // LDS of one contraction's blocks and CQ_PCH coefficient products for the Kronecker path
                     // with the class's own contraction (no eco), or -1: that schedule does not fit
    int o_kacc;      // LDS of the K accumulators (the four K blocks of a set quartet, summed over its

@@ -1348,22 +1348,24 @@ __device__ __forceinline__ double kron_m(int l1, int l2, int s, int g, const dou
    return v;
 }
 
-// One (set quartet, class) on the Kronecker path, four wavefronts.  LDS at
-// y.o_kron: Y = C M_ket^T [16 TM1][16 TN] (bra components x ket spherical
-// pairs, padded to whole tiles), then the two T tables.  W (the
-// workgroup's HBM slice, or LDS at y.o_kmap for a class whose layered
-// workspace fits in LDS) holds the maps of up to kron_ws / (one row's maps) rows:
-// all rows of the set quartet when they fit (built once), else one row at
-// a time (rebuilt per contraction).  Each wavefront keeps a 16-row band of
-// the spherical integrals (bra pairs x ket pairs) in its accumulators over
-// the rows, then makes the contraction's K updates (or writes its
-// integrals) as cq_one does.
+// One (set quartet, class) on the Kronecker path, four wavefronts, kb =
+// y.kron_kb contractions per pass.  LDS at y.o_kron: per contraction of the
+// pass a block of YB doubles (Y = C M_ket^T, bra components x ket spherical
+// pairs padded to whole 16x16 tiles; later the contraction's integrals),
+// then the two T tables; at y.o_kc (own contraction) the pass's contracted
+// blocks and coefficient products.  W (the workgroup's HBM slice, or LDS at
+// y.o_kmap) holds the maps of up to kron_ws / (one row's maps) rows: all
+// rows of the set quartet when they fit (built once), else one row at a
+// time (rebuilt per pass).  The pass's integral tiles (at most 16: four per
+// wavefront) stay in the wavefronts' accumulators over the rows; then they
+// go to LDS and to the K updates (or the integral output) of the pass's
+// contractions.
 template <int NT>
 __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, const DevTables &t, const BatchDev &b,
                         const KTargets &kt, const CqLayout &y, double *W, uint32_t gi, const char *tabs,
                         const double *co, const int *__restrict__ srcmap)
 {
-   static_assert(NT == 256, "one 16-row band of the integrals per wavefront, four of them");
+   static_assert(NT == 256, "four wavefronts");
    extern __shared__ double lds[];
    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
    const int la = cp.la, lb = cp.lb, lc = cp.lc, ld = cp.ld;
@@ -1374,19 +1376,21 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
    for (int e = la; e <= cp.LA; e++) nbra += d_nco(e);
    for (int f = lc; f <= cp.LC; f++) nket += d_nco(f);
    const int TM1 = (nbra + 15) >> 4, TN = (nscd + 15) >> 4, TM2 = (nsab + 15) >> 4;
-   const int YS = 16 * TN;
+   const int YS = 16 * TN, YB = 16 * max(TM1, TM2) * YS;
+   const int t1 = TM1 * TN, t2 = TM2 * TN;
    const int KS1 = (nket + 3) >> 2, KS2 = (nbra + 3) >> 2;
    const int nik = ioff3(ld + 1), nib = ioff3(lb + 1);
+   const int KB = y.kron_kb;
    double *Y = lds + y.o_kron;
-   double *Tk = Y + 16 * max(TM1, TM2) * YS, *Tb = Tk + nsd * nik;
+   double *Tk = Y + KB * YB, *Tb = Tk + nsd * nik;
    const int MS = nscd * nket + nsab * nbra;
    const int rc = y.kron_ws / MS;
    const double *c2s = (const double *)(tabs + CQ_TAB_C2S);
    const double *ca = c2s + c_c2s_off[la], *cb = c2s + c_c2s_off[lb], *cc = c2s + c_c2s_off[lc],
                 *cd = c2s + c_c2s_off[ld];
    const int sh = tp.cls_sh[cls_k];
-   const int nlb = tp.nl[1][(sh >> 4) & 15], nlc = tp.nl[2][(sh >> 8) & 15], nld = tp.nl[3][(sh >> 12) & 15];
-   const double symm[5] = {2.0, 1.0, 0.5, 0.25, 0.0};
+   const int nla = tp.nl[0][sh & 15], nlb = tp.nl[1][(sh >> 4) & 15], nlc = tp.nl[2][(sh >> 8) & 15],
+             nld = tp.nl[3][(sh >> 12) & 15];
    const GroupRec &g = b.groups[gi];
    const KsGroup ks = group_ks(b, tp, g, cls_k);
    const int nq = (int)(ks.flags >> 16);
@@ -1396,7 +1400,6 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
    const uint32_t nr = g.row1 - g.row0;
    const bool cached = nr <= (uint32_t)rc;
    // K accumulators in LDS (summed over the contractions before the atomics)
-   const int nla = tp.nl[0][sh & 15];
    const bool use_acc = y.o_kacc >= 0 && !expl && !out_mode && nq == nla * nlb * nlc * nld &&
                         kt.nspin * y.kacc <= y.kacc_cap;
    const int NA = nla * nsa, NB = nlb * nsb, NC = nlc * nsc, ND = nld * nsd;
@@ -1404,11 +1407,11 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
    if (use_acc)
       for (int i = tid; i < kt.nspin * y.kacc; i += NT) kap[i] = 0.0;
 
-   // the two maps of row r into slot m of the slice
+   // the two maps of row r into slot m of W
    auto build = [&](uint32_t r, int m) {
       double AB[3], CD[3];
       row_ab_cd(g, b.rows[r], b.data, t, AB, CD);
-      __syncthreads();   // the T tables are no longer read
+      __syncthreads();   // the T tables and the slot are no longer read
       for (int idx = tid; idx < nsd * nik + nsb * nib; idx += NT) {
          if (idx < nsd * nik) {
             Tk[idx] = kron_t(ld, idx / nik, idx % nik, CD, cd);
@@ -1431,46 +1434,56 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
    if (cached)
       for (uint32_t r = g.row0; r < g.row1; r++) build(r, (int)(r - g.row0));
 
-   for (int l = 0; l < nq; l++) {
+   for (int l0 = 0; l0 < nq; l0 += KB) {
+      const int nb = min(KB, nq - l0);
       kdbl4 acc[4];
 #pragma unroll
-      for (int tn = 0; tn < 4; tn++) acc[tn] = kdbl4{0.0, 0.0, 0.0, 0.0};
+      for (int i = 0; i < 4; i++) acc[i] = kdbl4{0.0, 0.0, 0.0, 0.0};
       for (uint32_t r = g.row0; r < g.row1; r++) {
          if (!cached) build(r, 0);
          __syncthreads();   // the maps are written, the previous Y is no longer read
          const double *Mk = W + (size_t)(cached ? r - g.row0 : 0) * MS, *Mb = Mk + nscd * nket;
-         const double *Cl;
+         const double *Cb;   // contraction l0 + l of the row at Cb + l ncp
          if (y.eco) {
             // contracted on the matrix cores (k_eco)
-            Cl = b.eco + (size_t)(r - b.r0) * b.eco_row + y.eco_off + (size_t)l * ncp;
+            Cb = b.eco + (size_t)(r - b.r0) * b.eco_row + y.eco_off + (size_t)l0 * ncp;
          } else {
             // the class's own contraction of the row's primitive blocks for
-            // contraction l, into LDS (as the layered path's, one contraction)
-            double *Cs = lds + y.o_kc, *Kp = Cs + ncp;
+            // the pass's contractions, into LDS (as the layered path's)
+            double *Cs = lds + y.o_kc, *Kp = Cs + KB * ncp;
             const RowRec row = b.rows[r];
             const uint32_t *IP = b.prm_ip + row.prm_off;
             const double *A0 = b.ac0 + (size_t)(row.prm_off - b.p0) * tp.ncp;
-            for (int j = tid; j < ncp; j += NT) Cs[j] = 0.0;
+            for (int j = tid; j < nb * ncp; j += NT) Cs[j] = 0.0;
             for (int pc = 0; pc < (int)row.n_prm; pc += CQ_PCH) {
                const int np = min(CQ_PCH, (int)row.n_prm - pc);
                __syncthreads();   // the previous step's products are no longer read
-               if (tid < CQ_PCH) Kp[tid] = tid < np ? kprod_lds(tp, co, cls_k, l, IP[pc + tid]) : 0.0;
+               for (int i = tid; i < nb * CQ_PCH; i += NT) {
+                  const int l = i / CQ_PCH, pp = i - l * CQ_PCH;
+                  Kp[i] = pp < np ? kprod_lds(tp, co, cls_k, l0 + l, IP[pc + pp]) : 0.0;
+               }
                __syncthreads();
                const double *A = A0 + (size_t)pc * tp.ncp;
                for (int j = tid; j < ncp; j += NT) {
                   const int src = srcmap[j];
-                  double sum = 0.0;
-                  for (int pp = 0; pp < np; pp++) sum += Kp[pp] * A[(size_t)pp * tp.ncp + src];
-                  Cs[j] += sum;
+                  double av[CQ_PCH];
+#pragma unroll
+                  for (int pp = 0; pp < CQ_PCH; pp++) av[pp] = pp < np ? A[(size_t)pp * tp.ncp + src] : 0.0;
+                  for (int l = 0; l < nb; l++) {
+                     double sum = 0.0;
+#pragma unroll
+                     for (int pp = 0; pp < CQ_PCH; pp++) sum += Kp[l * CQ_PCH + pp] * av[pp];
+                     Cs[l * ncp + j] += sum;
+                  }
                }
             }
             __syncthreads();
-            Cl = Cs;
+            Cb = Cs;
          }
-         // Y = C M_ket^T, wavefront wv the tiles wv, wv + 4, ... (one
-         // accumulator tile at a time: the integrals' four hold registers)
-         for (int tile = wv; tile < TM1 * TN; tile += 4) {
-            const int tm = tile / TN, tn = tile - tm * TN;
+         // Y_l = C_l M_ket^T: the pass's nb x TM1 x TN tiles over the wavefronts
+         for (int tile = wv; tile < nb * t1; tile += 4) {
+            const int l = tile / t1, u = tile - l * t1, tm = u / TN, tn = u - tm * TN;
+            const double *Cl = Cb + (size_t)l * ncp;
             const int bc = 16 * tm + (lane & 15), s = 16 * tn + (lane & 15);
             const bool bv = bc < nbra, sv = s < nscd;
             int e = la, ie = bc;
@@ -1497,77 +1510,88 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
                   f++;
                }
             }
+            double *Yl = Y + l * YB;
 #pragma unroll
-            for (int rr = 0; rr < 4; rr++) Y[(16 * tm + 4 * rr + (lane >> 4)) * YS + 16 * tn + (lane & 15)] = d[rr];
+            for (int rr = 0; rr < 4; rr++) Yl[(16 * tm + 4 * rr + (lane >> 4)) * YS + 16 * tn + (lane & 15)] = d[rr];
          }
          __syncthreads();
-         // integrals += M_bra Y, wavefront wv the 16-row band wv
-         if (wv < TM2) {
-            const int sab = 16 * wv + (lane & 15);
-            for (int k4 = 0; k4 < KS2; k4++) {
-               const int bk = 4 * k4 + (lane >> 4);
-               const double av = (sab < nsab && bk < nbra) ? Mb[sab * nbra + bk] : 0.0;
+         // integrals_l += M_bra Y_l: wavefront wv the pass's tiles wv, wv + 4, ... (at most 4)
 #pragma unroll
-               for (int tn = 0; tn < 4; tn++)
-                  if (tn < TN) acc[tn] = kron_mfma(av, Y[bk * YS + 16 * tn + (lane & 15)], acc[tn]);
+         for (int i = 0; i < 4; i++) {
+            const int tile = wv + 4 * i;
+            if (tile < nb * t2) {
+               const int l = tile / t2, u = tile - l * t2, tm = u / TN, tn = u - tm * TN;
+               const double *Yl = Y + l * YB + 16 * tn + (lane & 15);
+               const int sab = 16 * tm + (lane & 15);
+               for (int k4 = 0; k4 < KS2; k4++) {
+                  const int bk = 4 * k4 + (lane >> 4);
+                  const double av = (sab < nsab && bk < nbra) ? Mb[sab * nbra + bk] : 0.0;
+                  acc[i] = kron_mfma(av, Yl[bk * YS], acc[i]);
+               }
             }
          }
       }
-      // the integrals of contraction l into LDS (over Y), [bra pair][YS]
+      // the pass's integrals into LDS (over Y): contraction l at Y + l YB, [bra pair][YS]
       __syncthreads();
-      if (wv < TM2)
 #pragma unroll
-         for (int tn = 0; tn < 4; tn++)
-            if (tn < TN)
+      for (int i = 0; i < 4; i++) {
+         const int tile = wv + 4 * i;
+         if (tile < nb * t2) {
+            const int l = tile / t2, u = tile - l * t2, tm = u / TN, tn = u - tm * TN;
 #pragma unroll
-               for (int rr = 0; rr < 4; rr++)
-                  Y[(16 * wv + 4 * rr + (lane >> 4)) * YS + 16 * tn + (lane & 15)] = acc[tn][rr];
+            for (int rr = 0; rr < 4; rr++)
+               Y[l * YB + (16 * tm + 4 * rr + (lane >> 4)) * YS + 16 * tn + (lane & 15)] = acc[i][rr];
+         }
+      }
       __syncthreads();
-      if (use_acc) {
-         const int ia = l / (nlb * nlc * nld), ib = (l / (nlc * nld)) % nlb, ic = (l / nld) % nlc, id = l % nld;
-         size_t off[4];
-         ks_offsets(ks, nlb, nlc, nld, l, off);
-         kacc_add<NT>(kt, kap, y.kacc, off, ks.flags, ks.ld01, ks.ld23, ia, ib, ic, id, nsa, nsb, nsc, nsd, NA, NB, NC,
-                      ND, [&](int isa, int isb, int isc, int isd) {
-                         return Y[(isa * nsb + isb) * YS + isc * nsd + isd];
-                      });
-         continue;
-      }
-      // its K updates (or its integrals, in the TRA order)
-      size_t off[4] = {0, 0, 0, 0};
-      uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
-      if (expl) {
-         const KsRec x = b.ksx[ks.expl + l];
-         for (int q = 0; q < 4; q++) off[q] = x.off[q];
-         ld01 = x.ld01;
-         ld23 = x.ld23;
-         flags = x.flags;
-      } else if (!out_mode) {
-         ks_offsets(ks, nlb, nlc, nld, l, off);
-      }
-      const double fac = symm[(flags >> 4) & 7] * kt.hf_fac;
-      const unsigned ldac = ld01 & 0xffff, ldad = ld01 >> 16, ldbc = ld23 & 0xffff, ldbd = ld23 >> 16;
-      for (int tt = tid; tt < cp.Ns; tt += NT) {
-         const unsigned isa = tt / nsbcd, isb = (tt / nscd) % nsb, isc = (tt / nsd) % nsc, isd = tt % nsd;
-         const double v = Y[(tt / nscd) * YS + tt % nscd];
-         if (out_mode) {
-            double *O = kt.out + ((size_t)ks.base[0] | (size_t)ks.base[1] << 32);
-            const int ia = l / (nlb * nlc * nld), ib = (l / (nlc * nld)) % nlb, ic = (l / nld) % nlc, id = l % nld;
-            O[tra_index(ia, isa, ib, isb, ic, isc, id, isd, nlb, nlc, nld, nsa, nsb, nsc, nsd)] = v;
+      // their K updates (or their integrals, in the TRA order)
+      for (int l = 0; l < nb; l++) {
+         const int q = l0 + l;
+         const double *Il = Y + l * YB;
+         const int ia = q / (nlb * nlc * nld), ib = (q / (nlc * nld)) % nlb, ic = (q / nld) % nlc, id = q % nld;
+         if (use_acc) {
+            size_t off[4];
+            ks_offsets(ks, nlb, nlc, nld, q, off);
+            kacc_add<NT>(kt, kap, y.kacc, off, ks.flags, ks.ld01, ks.ld23, ia, ib, ic, id, nsa, nsb, nsc, nsd, NA, NB,
+                         NC, ND, [&](int isa, int isb, int isc, int isd) {
+                            return Il[(isa * nsb + isb) * YS + isc * nsd + isd];
+                         });
             continue;
          }
-         const size_t Iac = off[0] + blk_idx(flags & 1, ldac, isa, isc);
-         const size_t Iad = off[1] + blk_idx(flags & 2, ldad, isa, isd);
-         const size_t Ibc = off[2] + blk_idx(flags & 4, ldbc, isb, isc);
-         const size_t Ibd = off[3] + blk_idx(flags & 8, ldbd, isb, isd);
-         const double iabcd = -fac * v;
-         for (int sp = 0; sp < kt.nspin; sp++) {
-            const double *P = kt.P[sp];
-            double *K = kt.K[sp];
-            unsafeAtomicAdd(&K[Ibd], iabcd * P[Iac]);
-            unsafeAtomicAdd(&K[Ibc], iabcd * P[Iad]);
-            unsafeAtomicAdd(&K[Iad], iabcd * P[Ibc]);
-            unsafeAtomicAdd(&K[Iac], iabcd * P[Ibd]);
+         size_t off[4] = {0, 0, 0, 0};
+         uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
+         if (expl) {
+            const KsRec x = b.ksx[ks.expl + q];
+            for (int qq = 0; qq < 4; qq++) off[qq] = x.off[qq];
+            ld01 = x.ld01;
+            ld23 = x.ld23;
+            flags = x.flags;
+         } else if (!out_mode) {
+            ks_offsets(ks, nlb, nlc, nld, q, off);
+         }
+         const double fac = symm_code_fac((flags >> 4) & 7) * kt.hf_fac;
+         const unsigned ldac = ld01 & 0xffff, ldad = ld01 >> 16, ldbc = ld23 & 0xffff, ldbd = ld23 >> 16;
+         for (int tt = tid; tt < cp.Ns; tt += NT) {
+            const unsigned isa = tt / nsbcd, isb = (tt / nscd) % nsb, isc = (tt / nsd) % nsc, isd = tt % nsd;
+            const double v = Il[(tt / nscd) * YS + tt % nscd];
+            if (out_mode) {
+               double *O = kt.out + ((size_t)ks.base[0] | (size_t)ks.base[1] << 32);
+               O[tra_index(ia, isa, ib, isb, ic, isc, id, isd, nlb, nlc, nld, nsa, nsb, nsc, nsd)] = v;
+               continue;
+            }
+            const size_t Iac = off[0] + blk_idx(flags & 1, ldac, isa, isc);
+            const size_t Iad = off[1] + blk_idx(flags & 2, ldad, isa, isd);
+            const size_t Ibc = off[2] + blk_idx(flags & 4, ldbc, isb, isc);
+            const size_t Ibd = off[3] + blk_idx(flags & 8, ldbd, isb, isd);
+            const double iabcd = -fac * v;
+            for (int sp = 0; sp < kt.nspin; sp++) {
+               const double *P = kt.P[sp];
+               double *K = kt.K[sp];
+               unsafeAtomicAdd(&K[Ibd], iabcd * P[Iac]);
+               unsafeAtomicAdd(&K[Ibc], iabcd * P[Iad]);
+               unsafeAtomicAdd(&K[Iad], iabcd * P[Ibc]);
+               unsafeAtomicAdd(&K[Iac], iabcd * P[Ibd]);
+            }
          }
       }
    }
@@ -1581,11 +1605,12 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
 // set quartet on consecutive items: a type with few, large set quartets (an
 // spdf centre's 16^4 primitive quartets and 256 classes) still fills the
 // chip, and the set quartet's blocks are read by neighbouring workgroups
-// while they sit in L2.  at most 128 VGPRs (4 wavefronts per SIMD): the
-// kernel waits on memory, and more wavefronts in flight outweigh the few
-// values it then keeps in scratch
-template <int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_gqs(const ClassPlan *__restrict__ cps, const CqLayout *__restrict__ lays,
+// while they sit in L2.  WPE wavefronts per SIMD: at most 128 VGPRs for 4
+// (the kernel waits on memory, and more wavefronts in flight outweigh the
+// few values it then keeps in scratch); 2 (256 VGPRs, no spills) where the
+// workgroup's LDS leaves room for only two workgroups per CU anyway
+template <int NT, int WPE = 4>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_gqs(const ClassPlan *__restrict__ cps, const CqLayout *__restrict__ lays,
                                                     const int *__restrict__ srcmaps, const TypePlan *__restrict__ tpp,
                                                     DevTables t, BatchDev b, KTargets kt, double *gws, int gws_stride,
                                                     int cls_bytes, int part)
@@ -1727,49 +1752,53 @@ static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget_all, int kacc)
    y.o_I = off;
    if (y.reduce) off += need_I;
    y.o_src = 0;
-   // the Kronecker path (cq_kron) of a class whose layers do not fit: one
-   // contraction's bra components x ket spherical functions in LDS, the
-   // row's two maps in the workgroup's HBM slice
+   // the Kronecker path (cq_kron): per pass of kb contractions a block of
+   // bra components x ket spherical pairs each (whole 16x16 tiles), the T
+   // tables, with the class's own contraction (no eco) the pass's blocks and
+   // coefficient products of CQ_PCH primitive quartets, then the rows' maps
+   // (in LDS, or in the workgroup's HBM slice for a class whose layered
+   // workspace is there).  The layered path's LDS is free while a class
+   // takes this one: its region starts at 0.  kb: at most 16 integral tiles
+   // per pass (four per wavefront), at most 8, fewer where LDS is short
    int nbra = 0, nket = 0;
    for (int e = cp.la; e <= cp.LA; e++) nbra += nco(e);
    for (int f = cp.lc; f <= cp.LC; f++) nket += nco(f);
    const int nsab = (2 * cp.la + 1) * (2 * cp.lb + 1), nscd = (2 * cp.lc + 1) * (2 * cp.ld + 1);
-   // (the layered path's LDS is free while a class takes this one: its
-   // region starts at 0)
-   const int kron_lds = 16 * std::max((nbra + 15) / 16, (nsab + 15) / 16) * 16 * ((nscd + 15) / 16) +
-                        (2 * cp.ld + 1) * ioff3(cp.ld + 1) + (2 * cp.lb + 1) * ioff3(cp.lb + 1);
+   const int TM1 = (nbra + 15) / 16, TN = (nscd + 15) / 16, TM2 = (nsab + 15) / 16;
+   const int YB = 16 * std::max(TM1, TM2) * 16 * TN;
+   const int Tt = (2 * cp.ld + 1) * ioff3(cp.ld + 1) + (2 * cp.lb + 1) * ioff3(cp.lb + 1);
    const int maps = nscd * nket + nsab * nbra;
-   // with the class's own contraction (no eco): one contraction's blocks
-   // and the coefficient products of CQ_PCH primitive quartets, after the
-   // block and T tables
-   const int kcneed = cp.ncp + CQ_PCH;
    y.o_kron = 0;
    y.kron_ws = 0;
    y.o_kmap = -1;
    y.o_kc = -1;
    y.kron = 0;
+   y.kron_kb = 0;
    int kbytes = 0;
-   if (y.ws_global) {
-      // the maps of up to 8 rows in the workgroup's HBM slice, within the
-      // slice the layered path uses
-      if (kron_lds <= budget) {
+   for (int kb = std::max(1, std::min(std::min(8, 16 / (TM2 * TN)), nq)); kb >= 1; kb--) {
+      const int klds = kb * YB + Tt;
+      const int kcneed = kb * (cp.ncp + CQ_PCH);
+      if (y.ws_global) {
+         // the maps of up to 8 rows in the HBM slice the layered path uses
+         if (klds > budget) continue;
          y.kron_ws = std::max(1, std::min(8, y.lws / maps)) * maps;
-         kbytes = kron_lds;
-         if (kron_lds + kcneed <= budget) {
-            y.o_kc = kron_lds;
+         kbytes = klds;
+         if (klds + kcneed <= budget) {
+            y.o_kc = klds;
             kbytes += kcneed;
          }
-      }
-   } else {
-      // the maps of up to 8 rows in LDS, within the type's budget
-      const bool own = kron_lds + kcneed + maps <= budget;
-      const int base = kron_lds + (own ? kcneed : 0);
-      if (base + maps <= budget) {
-         y.o_kc = own ? kron_lds : -1;
+      } else {
+         // the maps of up to 8 rows in LDS, within the type's budget
+         const bool own = klds + kcneed + maps <= budget;
+         const int base = klds + (own ? kcneed : 0);
+         if (base + maps > budget) continue;
+         y.o_kc = own ? klds : -1;
          y.o_kmap = base;
          y.kron_ws = std::min(8, (budget - base) / maps) * maps;
          kbytes = base + y.kron_ws;
       }
+      y.kron_kb = kb;
+      break;
    }
    y.o_kacc = kcap ? std::max(off, kbytes) : -1;
    y.bytes = 8 * (std::max(off, kbytes) + kcap);
@@ -1871,7 +1900,12 @@ int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lay
    // ones, so that more wavefronts hide the memory latency; so does one with
    // a workspace in HBM or a class on the Kronecker path (`wide`: that path
    // takes four wavefronts)
-   if (total > 32 * 1024 || gws_stride > 0 || wide)
+   // (above a third of the CU's 160 KB of LDS two workgroups fit: the
+   // 256-VGPR variant loses no wavefronts)
+   if (total > 160 * 1024 / 3)
+      hipLaunchKernelGGL((k_gqs<256, 2>), dim3(grid), dim3(256), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
+                         b.tp, t, b, kt, gws, gws_stride, cls_bytes, part);
+   else if (total > 32 * 1024 || gws_stride > 0 || wide)
       hipLaunchKernelGGL(k_gqs<256>, dim3(grid), dim3(256), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
                          b.tp, t, b, kt, gws, gws_stride, cls_bytes, part);
    else
