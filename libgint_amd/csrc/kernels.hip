@@ -1607,8 +1607,8 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
 // chip, and the set quartet's blocks are read by neighbouring workgroups
 // while they sit in L2.  WPE wavefronts per SIMD: at most 128 VGPRs for 4
 // (the kernel waits on memory, and more wavefronts in flight outweigh the
-// few values it then keeps in scratch); 2 (256 VGPRs, no spills) where the
-// workgroup's LDS leaves room for only two workgroups per CU anyway
+// few values it then keeps in scratch); 3 or 2 (168 or 256 VGPRs) where the
+// workgroup's LDS leaves room for only that many workgroups per CU anyway
 template <int NT, int WPE = 4>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_gqs(const ClassPlan *__restrict__ cps, const CqLayout *__restrict__ lays,
                                                     const int *__restrict__ srcmaps, const TypePlan *__restrict__ tpp,
@@ -1900,10 +1900,13 @@ int launch_gqs(const TypePlan &tp, const ClassPlan *d_cps, const CqLayout *d_lay
    // ones, so that more wavefronts hide the memory latency; so does one with
    // a workspace in HBM or a class on the Kronecker path (`wide`: that path
    // takes four wavefronts)
-   // (above a third of the CU's 160 KB of LDS two workgroups fit: the
-   // 256-VGPR variant loses no wavefronts)
+   // (above a third of the CU's 160 KB of LDS two workgroups fit, above a
+   // quarter three: the variants with 256 and 168 VGPRs lose no wavefronts)
    if (total > 160 * 1024 / 3)
       hipLaunchKernelGGL((k_gqs<256, 2>), dim3(grid), dim3(256), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
+                         b.tp, t, b, kt, gws, gws_stride, cls_bytes, part);
+   else if (total > 160 * 1024 / 4)
+      hipLaunchKernelGGL((k_gqs<256, 3>), dim3(grid), dim3(256), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
                          b.tp, t, b, kt, gws, gws_stride, cls_bytes, part);
    else if (total > 32 * 1024 || gws_stride > 0 || wide)
       hipLaunchKernelGGL(k_gqs<256>, dim3(grid), dim3(256), total, (hipStream_t)stream, d_cps, d_lays, d_srcmaps,
