@@ -1445,8 +1445,17 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
          const double *Mk = W + (size_t)(cached ? r - g.row0 : 0) * MS, *Mb = Mk + nscd * nket;
          const double *Cb;   // contraction l0 + l of the row at Cb + l ncp
          if (y.eco) {
-            // contracted on the matrix cores (k_eco)
+            // contracted on the matrix cores (k_eco): the pass's blocks are
+            // contiguous in the contraction buffer; where the LDS has room
+            // (o_kc) they are copied there with coalesced loads, once per
+            // pass, instead of each tile lane loading its own 32 bytes
             Cb = b.eco + (size_t)(r - b.r0) * b.eco_row + y.eco_off + (size_t)l0 * ncp;
+            if (y.o_kc >= 0) {
+               double *Cs = lds + y.o_kc;
+               for (int j = tid; j < nb * ncp; j += NT) Cs[j] = Cb[j];
+               __syncthreads();
+               Cb = Cs;
+            }
          } else {
             // the class's own contraction of the row's primitive blocks for
             // the pass's contractions, into LDS (as the layered path's)
@@ -1808,12 +1817,15 @@ static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget_all, int kacc)
 void make_cq_plan(const TypePlan &tp, const std::vector<ClassPlan> &cps, std::vector<CqLayout> &lays,
                   std::vector<int> &srcmaps, int *lds_bytes, int *gws_stride)
 {
-   // the budget of the type: what its largest class needs, within 64 KB
+   // the budget of the type: what its largest class needs, within 50 KB
+   // per workgroup all told (three workgroups per CU: more wavefronts to
+   // hide the latency; a 64 KB budget, two per CU, was 8% slower on the
+   // 128-water workload and 1% on the 64-water + spdf one)
    int budget = 512;
    for (int k = 0; k < tp.nclass; k++) {
       if (tq_has(tp.cls_code[k])) continue;
       const int need = 3 * cps[k].ycap + cps[k].Nc + CQ_PCH + tp.cls_nl[k] * cps[k].Ns + CQ_PCH;
-      budget = std::max(budget, std::min(need, 8192 - 160 - tp.nco_tot - 4 - CQ_TAB_BYTES / 8 - CQ_PLIST_BYTES / 8 - 2));
+      budget = std::max(budget, std::min(need, 6400 - 160 - tp.nco_tot - 4 - CQ_TAB_BYTES / 8 - CQ_PLIST_BYTES / 8 - 2));
    }
    lays.clear();
    srcmaps.clear();
