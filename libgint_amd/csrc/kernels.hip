@@ -823,52 +823,66 @@ __device__ __forceinline__ unsigned blk_idx(bool T, unsigned ld, unsigned row_i,
 // symmetry factor of a K layout code: 2, 1, 1/2, 1/4, or 0 (code 4)
 __device__ __forceinline__ double symm_code_fac(uint32_t c) { return c >= 4 ? 0.0 : (c == 0 ? 2.0 : (c == 1 ? 1.0 : (c == 2 ? 0.5 : 0.25))); }
 
-// The K updates of one contraction (ia, ib, ic, id), summed into the set
-// quartet's K accumulators in LDS instead of four atomics per integral.
-// Per spin the accumulators are the four K blocks, [ac: NA x NC][ad: NA x ND]
-// [bc: NB x NC][bd: NB x ND], rows and columns (contraction, spherical
-// function) of their centres.  Each element is the sum, over the two other
-// spherical indices, of the integrals times the P block it pairs with (the
-// reference's updates K_bd += I P_ac, K_bc += I P_ad, K_ad += I P_bc,
-// K_ac += I P_bd per integral, src/compute_KS.cpp:8-117, regrouped; the
-// factor -f is applied at the flush).  Thread t takes the same elements for
-// every contraction, so the contractions a thread adds one after the other
-// need no synchronisation.
+// K accumulators: the K updates of a set quartet's contractions summed in
+// LDS instead of four atomics per integral.  Per spin the accumulators are
+// the four K blocks, [ac: NA x NC][ad: NA x ND][bc: NB x NC][bd: NB x ND],
+// rows and columns (contraction, spherical function) of their centres.  Each
+// element is the sum, over the two other spherical indices, of the
+// integrals times the P block it pairs with (the reference's updates
+// K_bd += I P_ac, K_bc += I P_ad, K_ad += I P_bc, K_ac += I P_bd per
+// integral, src/compute_KS.cpp:8-117, regrouped; the factor -f is applied
+// at the flush).
+//
+// kacc_add for the nb contractions l0 .. l0 + nb - 1 of a pass at once:
+// the (contraction, element) items spread over all threads, each sum added
+// to its accumulator with an LDS atomic (contractions of a pass that share
+// two of the indices meet on the same element).  I(l, isa, isb, isc, isd):
+// the integrals of contraction l0 + l.
 template <int NT, typename IntegralFn>
-__device__ __forceinline__ void kacc_add(const KTargets &kt, double *acc, int kacc, const size_t off[4], uint32_t flags,
-                                         uint32_t ld01, uint32_t ld23, int ia, int ib, int ic, int id, int nsa, int nsb,
-                                         int nsc, int nsd, int NA, int NB, int NC, int ND, IntegralFn I)
+__device__ __forceinline__ void kacc_add_pass(const KTargets &kt, double *acc, int kacc, const KsGroup &ks, int l0,
+                                              int nb, int nlb, int nlc, int nld, int nsa, int nsb, int nsc, int nsd,
+                                              int NA, int NB, int NC, int ND, IntegralFn I)
 {
-   const unsigned ldac = ld01 & 0xffff, ldad = ld01 >> 16, ldbc = ld23 & 0xffff, ldbd = ld23 >> 16;
-   const bool Tac = flags & 1, Tad = flags & 2, Tbc = flags & 4, Tbd = flags & 8;
+   const unsigned ldac = ks.ld01 & 0xffff, ldad = ks.ld01 >> 16, ldbc = ks.ld23 & 0xffff, ldbd = ks.ld23 >> 16;
+   const bool Tac = ks.flags & 1, Tad = ks.flags & 2, Tbc = ks.flags & 4, Tbd = ks.flags & 8;
    const int n_bd = nsb * nsd, n_bc = nsb * nsc, n_ad = nsa * nsd, n_ac = nsa * nsc;
    const int ntask = n_bd + n_bc + n_ad + n_ac;
-   for (int t = threadIdx.x; t < ntask; t += NT) {
+   for (int it = threadIdx.x; it < nb * ntask; it += NT) {
+      const int l = it / ntask, t = it - l * ntask, q = l0 + l;
+      const int ia = q / (nlb * nlc * nld), ib = (q / (nlc * nld)) % nlb, ic = (q / nld) % nlc, id = q % nld;
+      size_t off[4];
+      ks_offsets(ks, nlb, nlc, nld, q, off);
       for (int sp = 0; sp < kt.nspin; sp++) {
          const double *P = kt.P[sp];
          double *A = acc + sp * kacc;
          double sum = 0.0;
+         int dst;
          if (t < n_bd) {
             const int isb = t / nsd, isd = t - isb * nsd;
             for (int isa = 0; isa < nsa; isa++)
-               for (int isc = 0; isc < nsc; isc++) sum += I(isa, isb, isc, isd) * P[off[0] + blk_idx(Tac, ldac, isa, isc)];
-            A[NA * NC + NA * ND + NB * NC + (ib * nsb + isb) * ND + id * nsd + isd] += sum;
+               for (int isc = 0; isc < nsc; isc++)
+                  sum += I(l, isa, isb, isc, isd) * P[off[0] + blk_idx(Tac, ldac, isa, isc)];
+            dst = NA * NC + NA * ND + NB * NC + (ib * nsb + isb) * ND + id * nsd + isd;
          } else if (t < n_bd + n_bc) {
             const int u = t - n_bd, isb = u / nsc, isc = u - isb * nsc;
             for (int isa = 0; isa < nsa; isa++)
-               for (int isd = 0; isd < nsd; isd++) sum += I(isa, isb, isc, isd) * P[off[1] + blk_idx(Tad, ldad, isa, isd)];
-            A[NA * NC + NA * ND + (ib * nsb + isb) * NC + ic * nsc + isc] += sum;
+               for (int isd = 0; isd < nsd; isd++)
+                  sum += I(l, isa, isb, isc, isd) * P[off[1] + blk_idx(Tad, ldad, isa, isd)];
+            dst = NA * NC + NA * ND + (ib * nsb + isb) * NC + ic * nsc + isc;
          } else if (t < n_bd + n_bc + n_ad) {
             const int u = t - n_bd - n_bc, isa = u / nsd, isd = u - isa * nsd;
             for (int isb = 0; isb < nsb; isb++)
-               for (int isc = 0; isc < nsc; isc++) sum += I(isa, isb, isc, isd) * P[off[2] + blk_idx(Tbc, ldbc, isb, isc)];
-            A[NA * NC + (ia * nsa + isa) * ND + id * nsd + isd] += sum;
+               for (int isc = 0; isc < nsc; isc++)
+                  sum += I(l, isa, isb, isc, isd) * P[off[2] + blk_idx(Tbc, ldbc, isb, isc)];
+            dst = NA * NC + (ia * nsa + isa) * ND + id * nsd + isd;
          } else {
             const int u = t - n_bd - n_bc - n_ad, isa = u / nsc, isc = u - isa * nsc;
             for (int isb = 0; isb < nsb; isb++)
-               for (int isd = 0; isd < nsd; isd++) sum += I(isa, isb, isc, isd) * P[off[3] + blk_idx(Tbd, ldbd, isb, isd)];
-            A[(ia * nsa + isa) * NC + ic * nsc + isc] += sum;
+               for (int isd = 0; isd < nsd; isd++)
+                  sum += I(l, isa, isb, isc, isd) * P[off[3] + blk_idx(Tbd, ldbd, isb, isd)];
+            dst = (ia * nsa + isa) * NC + ic * nsc + isc;
          }
+         atomicAdd(&A[dst], sum);
       }
    }
 }
@@ -1143,18 +1157,11 @@ __device__ void cq_one(const ClassPlan &cp, int cls_k, const TypePlan &tp, const
          continue;
       }
       if (use_acc) {
-         // the batch's contractions into the K accumulators, one after the other
-         for (int l = 0; l < nb; l++) {
-            const int q = l0 + l;
-            const int ia = q / (nlb * nlc * nld), ib = (q / (nlc * nld)) % nlb, ic = (q / nld) % nlc, id = q % nld;
-            size_t off[4];
-            ks_offsets(ks, nlb, nlc, nld, q, off);
-            const double *Il = WS + l * lws + cap;
-            kacc_add<NT>(kt, acc, y.kacc, off, ks.flags, ks.ld01, ks.ld23, ia, ib, ic, id, nsa, nsb, nsc, nsd, NA, NB,
-                         NC, ND, [&](int isa, int isb, int isc, int isd) {
-                            return Il[((isa * nsb + isb) * nsc + isc) * nsd + isd];
-                         });
-         }
+         // the batch's contractions into the K accumulators
+         kacc_add_pass<NT>(kt, acc, y.kacc, ks, l0, nb, nlb, nlc, nld, nsa, nsb, nsc, nsd, NA, NB, NC, ND,
+                           [&](int l, int isa, int isb, int isc, int isd) {
+                              return WS[l * lws + cap + ((isa * nsb + isb) * nsc + isc) * nsd + isd];
+                           });
          continue;
       }
       // one contraction at a time: 4 atomic updates per spherical integral
@@ -1554,19 +1561,17 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
       }
       __syncthreads();
       // their K updates (or their integrals, in the TRA order)
+      if (use_acc) {
+         kacc_add_pass<NT>(kt, kap, y.kacc, ks, l0, nb, nlb, nlc, nld, nsa, nsb, nsc, nsd, NA, NB, NC, ND,
+                           [&](int l, int isa, int isb, int isc, int isd) {
+                              return Y[l * YB + (isa * nsb + isb) * YS + isc * nsd + isd];
+                           });
+         continue;
+      }
       for (int l = 0; l < nb; l++) {
          const int q = l0 + l;
          const double *Il = Y + l * YB;
          const int ia = q / (nlb * nlc * nld), ib = (q / (nlc * nld)) % nlb, ic = (q / nld) % nlc, id = q % nld;
-         if (use_acc) {
-            size_t off[4];
-            ks_offsets(ks, nlb, nlc, nld, q, off);
-            kacc_add<NT>(kt, kap, y.kacc, off, ks.flags, ks.ld01, ks.ld23, ia, ib, ic, id, nsa, nsb, nsc, nsd, NA, NB,
-                         NC, ND, [&](int isa, int isb, int isc, int isd) {
-                            return Il[(isa * nsb + isb) * YS + isc * nsd + isd];
-                         });
-            continue;
-         }
          size_t off[4] = {0, 0, 0, 0};
          uint32_t ld01 = ks.ld01, ld23 = ks.ld23, flags = ks.flags;
          if (expl) {
