@@ -137,9 +137,9 @@ void libgint_amd_timing_detail(void *handle, int cat, double out[256]);
    kernel of a chunk on the handle's stream, one after the other (per-kernel
    durations without overlap, for profiling; results are unchanged); bit 2:
    wait for the work of each quartet type and report its completion on
-   stderr (progress of long K-builds; results are unchanged); bit 3: resident
-   plans send every class that can take it to the Kronecker HRR, whatever
-   the autotuner chose, with the matrix-core contraction where the class
+   stderr (progress of long K-builds; results are unchanged); bit 3: every
+   class that can take it goes to the Kronecker HRR, whatever the autotuner
+   chose (resident plans) or the default rule (call streams), with the matrix-core contraction where the class
    has one (else its own); bit 4: the same with the class's own contraction
    where it fits (tests of that path; results are unchanged) */
 void libgint_amd_set_debug(void *handle, int flags);

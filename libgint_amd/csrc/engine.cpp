@@ -819,7 +819,7 @@ void Engine::plan_execute()
    if (!prepare_exec(plan_dev_, plan_max_index_, w, kt, FP_size_)) return;
    // (not while an integral-output array is set: its classes write there)
    if (autotune_on_ && !plan_tuned_ && !d_out_) autotune(w, kt);
-   if (debug_ & 24) force_kron((debug_ & 16) != 0);
+   if (debug_ & 24) force_kron(plan_dev_, (debug_ & 16) != 0);
    mixed_mode_ = 0;
    if (mixed_tau_ > 0.0 && !d_out_) {
       ensure_mixed();
@@ -1417,7 +1417,9 @@ void Engine::execute(std::vector<TypeDev> &devs, uint64_t max_index)
 {
    RunBufs w{};
    KTargets kt{};
-   if (prepare_exec(devs, max_index, w, kt, FP_size_)) launch_all(devs, w, kt);
+   if (!prepare_exec(devs, max_index, w, kt, FP_size_)) return;
+   if (debug_ & 24) force_kron(devs, (debug_ & 16) != 0);
+   launch_all(devs, w, kt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1577,14 +1579,14 @@ void Engine::apply_schedule(TypeInfo &ti, int k, int v)
    y.kron = v >= 3 ? 1 : 0;
 }
 
-// debug bits 3 and 4: every class of the plan that can take the Kronecker
-// path takes it, whatever the schedule: bit 3 with the matrix-core
+// debug bits 3 and 4: every class (of a plan, or of the call stream's
+// types) that can take the Kronecker path takes it, whatever the schedule: bit 3 with the matrix-core
 // contraction where the class has a place in the contraction buffer (else
 // its own), bit 4 with its own contraction where that fits (else the matrix
 // cores)
-void Engine::force_kron(bool own_first)
+void Engine::force_kron(std::vector<TypeDev> &devs, bool own_first)
 {
-   for (auto &D : plan_dev_) {
+   for (auto &D : devs) {
       if (D.chunks.empty()) continue;
       TypeInfo &ti = types_[D.type];
       bool any = false;

@@ -169,7 +169,7 @@ class Engine {
    void timing_detail(int cat, double out[256]);
    void reset_counters();
    void set_debug(int flags) { debug_ = flags; }
-   void force_kron(bool own_first);
+   void force_kron(std::vector<TypeDev> &devs, bool own_first);
    void set_graph(bool on) { graph_on_ = on; }
    void set_autotune(bool on) { autotune_on_ = on; }
    void set_mixed_precision(double tau) { mixed_tau_ = tau > 0.0 ? tau : 0.0; }

@@ -47,3 +47,25 @@ def test_kron_path_against_oracle(n_tm, seed):
         lib.libgint_amd_destroy_handle(h)
     for Kf in Ks[1:]:
         assert np.max(np.abs(Ks[0] - Kf)) <= KTOL_ABS + KTOL_REL * kmax
+
+
+def test_kron_path_integral_output():
+    """the integral output (the reference's TRA layout) of every class of a
+    call stream with an spdf centre, through the Kronecker path forced as
+    above (bits 3 and 4), against the oracle element by element"""
+    from libgint_amd import driver as D, load
+
+    lib = load()
+    o = oracle()
+    s = D.System(n_water=1, n_tm=1, box_A=6.0, image_shells=1, np_max=2, np_tm=2, seed=26)
+    st = s.record()
+    Io, ho = D.integrals(o, "oracle_", s, st)
+    o.oracle_destroy_handle(ho)
+    scale = np.max(np.abs(Io))
+    for debug in (8, 16):
+        Ip, hp = D.integrals(lib, "libgint_", s, st, debug=debug)
+        lib.libgint_amd_destroy_handle(hp)
+        err = np.max(np.abs(Ip - Io))
+        print("debug", debug, "integrals", Io.size, "max|I|", scale, "max|dI|", err, flush=True)
+        assert Io.size > 0 and scale > 0
+        assert err <= KTOL_ABS + KTOL_REL * scale, (debug, err)
