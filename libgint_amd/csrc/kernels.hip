@@ -1356,10 +1356,10 @@ __device__ __forceinline__ double kron_m(int l1, int l2, int s, int g, const dou
 }
 
 // One (set quartet, class) on the Kronecker path, four wavefronts, kb =
-// y.kron_kb contractions per pass.  LDS at y.o_kron: per contraction of the
-// pass a block of YB doubles (Y = C M_ket^T, bra components x ket spherical
-// pairs padded to whole 16x16 tiles; later the contraction's integrals),
-// then the two T tables; at y.o_kc (own contraction) the pass's contracted
+// y.kron_kb contractions per pass.  LDS at y.o_kron: kb blocks of YB
+// doubles (first Y = C M_ket^T for the pass's contractions stacked, bra
+// components x ket spherical pairs; then the contractions' integrals, one
+// block each), then the two T tables; at y.o_kc (own contraction) the pass's contracted
 // blocks and coefficient products.  W (the workgroup's HBM slice, or LDS at
 // y.o_kmap) holds the maps of up to kron_ws / (one row's maps) rows: all
 // rows of the set quartet when they fit (built once), else one row at a
@@ -1384,7 +1384,7 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
    for (int f = lc; f <= cp.LC; f++) nket += d_nco(f);
    const int TM1 = (nbra + 15) >> 4, TN = (nscd + 15) >> 4, TM2 = (nsab + 15) >> 4;
    const int YS = 16 * TN, YB = 16 * max(TM1, TM2) * YS;
-   const int t1 = TM1 * TN, t2 = TM2 * TN;
+   const int t2 = TM2 * TN;
    const int KS1 = (nket + 3) >> 2, KS2 = (nbra + 3) >> 2;
    const int nik = ioff3(ld + 1), nib = ioff3(lb + 1);
    const int KB = y.kron_kb;
@@ -1496,12 +1496,16 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
             __syncthreads();
             Cb = Cs;
          }
-         // Y_l = C_l M_ket^T: the pass's nb x TM1 x TN tiles over the wavefronts
-         for (int tile = wv; tile < nb * t1; tile += 4) {
-            const int l = tile / t1, u = tile - l * t1, tm = u / TN, tn = u - tm * TN;
+         // Y = [C_l0; C_l1; ...] M_ket^T: the pass's contractions stacked
+         // (row m = l nbra + bc: whole 16-row tiles over all of them, not
+         // per contraction), its TMS x TN tiles over the wavefronts
+         const int TMS = (nb * nbra + 15) >> 4;
+         for (int tile = wv; tile < TMS * TN; tile += 4) {
+            const int tm = tile / TN, tn = tile - tm * TN;
+            const int m = 16 * tm + (lane & 15), s = 16 * tn + (lane & 15);
+            const bool mv = m < nb * nbra, sv = s < nscd;
+            const int l = mv ? m / nbra : 0, bc = mv ? m - l * nbra : 0;
             const double *Cl = Cb + (size_t)l * ncp;
-            const int bc = 16 * tm + (lane & 15), s = 16 * tn + (lane & 15);
-            const bool bv = bc < nbra, sv = s < nscd;
             int e = la, ie = bc;
             while (e < cp.LA && ie >= d_nco(e)) {
                ie -= d_nco(e);
@@ -1517,7 +1521,7 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
             kdbl4 d = {0.0, 0.0, 0.0, 0.0};
             for (int gk = lane >> 4; gk < 4 * KS1; gk += 4) {
                const bool gv = gk < nket;
-               const double av = (bv && gv) ? Cl[cp.cpoff[e][f] + ie * d_nco(f) + jf] : 0.0;
+               const double av = (mv && gv) ? Cl[cp.cpoff[e][f] + ie * d_nco(f) + jf] : 0.0;
                const double bm = (gv && sv) ? Mks[gk] : 0.0;
                d = kron_mfma(av, bm, d);
                jf += 4;
@@ -1526,9 +1530,8 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
                   f++;
                }
             }
-            double *Yl = Y + l * YB;
 #pragma unroll
-            for (int rr = 0; rr < 4; rr++) Yl[(16 * tm + 4 * rr + (lane >> 4)) * YS + 16 * tn + (lane & 15)] = d[rr];
+            for (int rr = 0; rr < 4; rr++) Y[(16 * tm + 4 * rr + (lane >> 4)) * YS + 16 * tn + (lane & 15)] = d[rr];
          }
          __syncthreads();
          // integrals_l += M_bra Y_l: wavefront wv the pass's tiles wv, wv + 4, ... (at most 4)
@@ -1537,12 +1540,15 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
             const int tile = wv + 4 * i;
             if (tile < nb * t2) {
                const int l = tile / t2, u = tile - l * t2, tm = u / TN, tn = u - tm * TN;
-               const double *Yl = Y + l * YB + 16 * tn + (lane & 15);
+               const double *Yl = Y + (size_t)l * nbra * YS + 16 * tn + (lane & 15);
                const int sab = 16 * tm + (lane & 15);
                for (int k4 = 0; k4 < KS2; k4++) {
                   const int bk = 4 * k4 + (lane >> 4);
-                  const double av = (sab < nsab && bk < nbra) ? Mb[sab * nbra + bk] : 0.0;
-                  acc[i] = kron_mfma(av, Yl[bk * YS], acc[i]);
+                  // (rows past nbra belong to the next contraction or are unwritten)
+                  const bool kv = bk < nbra;
+                  const double av = (sab < nsab && kv) ? Mb[sab * nbra + bk] : 0.0;
+                  const double yv = kv ? Yl[bk * YS] : 0.0;
+                  acc[i] = kron_mfma(av, yv, acc[i]);
                }
             }
          }
