@@ -1687,8 +1687,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       }
       __syncthreads();
       double *WS = y.ws_global ? gws + (size_t)blockIdx.x * gws_stride : lds_base() + y.o_ws;
-      // the Kronecker path where it applies: contraction on the matrix
-      // cores, layers beyond LDS, and no class-dependent screening to add
+      // the Kronecker path where the class's schedule takes it (four
+      // wavefronts, its blocks contracted on the matrix cores or by itself)
+      // and the chunk has no class-dependent screening to add; else the
+      // layered one
       bool kron = false;
       if constexpr (NT == 256)
          kron = y.kron && y.kron_ws && (y.eco || y.o_kc >= 0) && min(b.fix_count[0], b.fix_cap) == 0;
