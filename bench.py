@@ -118,7 +118,7 @@ def serving(a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--stride", type=int, default=1, help="every n-th set quartet of the workload (1: all)")
