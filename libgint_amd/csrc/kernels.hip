@@ -1359,7 +1359,8 @@ __device__ __forceinline__ double kron_m(int l1, int l2, int s, int g, const dou
 // y.kron_kb contractions per pass.  LDS at y.o_kron: kb blocks of YB
 // doubles (first Y = C M_ket^T for the pass's contractions stacked, bra
 // components x ket spherical pairs; then the contractions' integrals, one
-// block each), then the two T tables; at y.o_kc (own contraction) the pass's contracted
+// block each; the second product takes the pass's contractions side by
+// side), then the two T tables; at y.o_kc (own contraction) the pass's contracted
 // blocks and coefficient products.  W (the workgroup's HBM slice, or LDS at
 // y.o_kmap) holds the maps of up to kron_ws / (one row's maps) rows: all
 // rows of the set quartet when they fit (built once), else one row at a
@@ -1384,7 +1385,6 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
    for (int f = lc; f <= cp.LC; f++) nket += d_nco(f);
    const int TM1 = (nbra + 15) >> 4, TN = (nscd + 15) >> 4, TM2 = (nsab + 15) >> 4;
    const int YS = 16 * TN, YB = 16 * max(TM1, TM2) * YS;
-   const int t2 = TM2 * TN;
    const int KS1 = (nket + 3) >> 2, KS2 = (nbra + 3) >> 2;
    const int nik = ioff3(ld + 1), nib = ioff3(lb + 1);
    const int KB = y.kron_kb;
@@ -1443,6 +1443,7 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
 
    for (int l0 = 0; l0 < nq; l0 += KB) {
       const int nb = min(KB, nq - l0);
+      const int TNS = (nb * nscd + 15) >> 4;   // integral tile columns of the pass
       kdbl4 acc[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) acc[i] = kdbl4{0.0, 0.0, 0.0, 0.0};
@@ -1534,20 +1535,25 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
             for (int rr = 0; rr < 4; rr++) Y[(16 * tm + 4 * rr + (lane >> 4)) * YS + 16 * tn + (lane & 15)] = d[rr];
          }
          __syncthreads();
-         // integrals_l += M_bra Y_l: wavefront wv the pass's tiles wv, wv + 4, ... (at most 4)
+         // integrals_l += M_bra Y_l for the pass's contractions side by side
+         // (column n = l nscd + scd: whole 16-column tiles over all of them),
+         // wavefront wv the tiles wv, wv + 4, ... (at most 4)
 #pragma unroll
          for (int i = 0; i < 4; i++) {
             const int tile = wv + 4 * i;
-            if (tile < nb * t2) {
-               const int l = tile / t2, u = tile - l * t2, tm = u / TN, tn = u - tm * TN;
-               const double *Yl = Y + (size_t)l * nbra * YS + 16 * tn + (lane & 15);
+            if (tile < TM2 * TNS) {
+               const int tm = tile / TNS, tn = tile - tm * TNS;
+               const int n = 16 * tn + (lane & 15);
+               const bool nv = n < nb * nscd;
+               const int l = nv ? n / nscd : 0, scd = nv ? n - l * nscd : 0;
+               const double *Yl = Y + (size_t)l * nbra * YS + scd;
                const int sab = 16 * tm + (lane & 15);
                for (int k4 = 0; k4 < KS2; k4++) {
                   const int bk = 4 * k4 + (lane >> 4);
                   // (rows past nbra belong to the next contraction or are unwritten)
                   const bool kv = bk < nbra;
                   const double av = (sab < nsab && kv) ? Mb[sab * nbra + bk] : 0.0;
-                  const double yv = kv ? Yl[bk * YS] : 0.0;
+                  const double yv = (kv && nv) ? Yl[bk * YS] : 0.0;
                   acc[i] = kron_mfma(av, yv, acc[i]);
                }
             }
@@ -1558,11 +1564,14 @@ __device__ void cq_kron(const ClassPlan &cp, int cls_k, const TypePlan &tp, cons
 #pragma unroll
       for (int i = 0; i < 4; i++) {
          const int tile = wv + 4 * i;
-         if (tile < nb * t2) {
-            const int l = tile / t2, u = tile - l * t2, tm = u / TN, tn = u - tm * TN;
+         if (tile < TM2 * TNS) {
+            const int tm = tile / TNS, tn = tile - tm * TNS;
+            const int n = 16 * tn + (lane & 15);
+            if (n < nb * nscd) {
+               const int l = n / nscd, scd = n - l * nscd;
 #pragma unroll
-            for (int rr = 0; rr < 4; rr++)
-               Y[l * YB + (16 * tm + 4 * rr + (lane >> 4)) * YS + 16 * tn + (lane & 15)] = acc[i][rr];
+               for (int rr = 0; rr < 4; rr++) Y[l * YB + (16 * tm + 4 * rr + (lane >> 4)) * YS + scd] = acc[i][rr];
+            }
          }
       }
       __syncthreads();
@@ -1781,7 +1790,8 @@ static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget_all, int kacc)
    // (in LDS, or in the workgroup's HBM slice for a class whose layered
    // workspace is there).  The layered path's LDS is free while a class
    // takes this one: its region starts at 0.  kb: at most 16 integral tiles
-   // per pass (four per wavefront), at most 8, fewer where LDS is short
+   // per pass (four per wavefront; the pass's contractions side by side in
+   // the tile columns), at most 16, fewer where LDS is short
    int nbra = 0, nket = 0;
    for (int e = cp.la; e <= cp.LA; e++) nbra += nco(e);
    for (int f = cp.lc; f <= cp.LC; f++) nket += nco(f);
@@ -1797,7 +1807,10 @@ static CqLayout cq_layout(const ClassPlan &cp, int nq, int budget_all, int kacc)
    y.kron = 0;
    y.kron_kb = 0;
    int kbytes = 0;
-   for (int kb = std::max(1, std::min(std::min(8, 16 / (TM2 * TN)), nq)); kb >= 1; kb--) {
+   // (kb: the pass's integral tiles, TM2 x ceil(kb nscd / 16), at most 16)
+   int kb0 = std::max(1, std::min(16, nq));
+   while (kb0 > 1 && TM2 * ((kb0 * nscd + 15) / 16) > 16) kb0--;
+   for (int kb = kb0; kb >= 1; kb--) {
       const int klds = kb * YB + Tt;
       const int kcneed = kb * (cp.ncp + CQ_PCH);
       if (y.ws_global) {
